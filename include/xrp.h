@@ -1,0 +1,180 @@
+/*
+ * xrp.h - C ABI of libxrp, the MI355X (gfx950) X-ray projection renderer.
+ *
+ * Drop-in boundary for the reference renderer's per-pixel line-integral
+ * path (igrega348/xray_projection_render).  Plain C types only; every
+ * pointer is caller-owned unless a function says otherwise.  Every entry
+ * point returns an xrp_status (0 = success); xrp_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Replaces:
+ *   - RenderVolumeProjectionsCUDA   cuda_backend.h:101-112  -> xrp_render_volume_projections
+ *   - the CPU render loop           main.go:429-471        -> xrp_render / xrp_render_device
+ *   - computeCameraFromAngles       main.go:226-239        -> xrp_camera_from_angles
+ *   - RenderProjections / FreeString api.go:82-180          -> same names (JSON in/out)
+ */
+#ifndef XRP_H
+#define XRP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XRP_ABI_VERSION 1
+
+/* Status codes.  1 and 2 keep the meaning of cuda_backend.cu:95-100. */
+typedef enum {
+  XRP_OK = 0,
+  XRP_ERR_NULL = 1,        /* a required pointer is NULL */
+  XRP_ERR_DIMS = 2,        /* non-positive size / resolution / view count */
+  XRP_ERR_DEVICE = 3,      /* HIP runtime failure (message has the HIP error) */
+  XRP_ERR_ALLOC = 4,       /* device or host allocation failed */
+  XRP_ERR_SCENE = 10,      /* object description rejected (reference FromMap rules) */
+  XRP_ERR_DEFORMATION = 11,/* deformation description rejected */
+  XRP_ERR_ARG = 12,        /* invalid parameter value (ds <= 0, unknown enum, ...) */
+  XRP_ERR_UNSUPPORTED = 13,/* valid for the reference but not for this build */
+  XRP_ERR_IO = 14          /* file could not be read or written */
+} xrp_status;
+
+/* Integrators: main.go:144-154 and main.go:159-199. */
+typedef enum { XRP_INTEGRATE_SIMPLE = 0, XRP_INTEGRATE_HIERARCHICAL = 1 } xrp_integrator;
+
+/* Precision of the density evaluation.
+ *  AUTO    - FP64 for scenes holding a voxel grid, GUARDED otherwise.
+ *  GUARDED - FP32 membership tests with guard bands; samples inside a band
+ *            are re-evaluated in FP64.  Densities and the integral are FP64.
+ *  FP64    - every sample evaluated in FP64 in the reference's order. */
+typedef enum {
+  XRP_PRECISION_AUTO = 0,
+  XRP_PRECISION_GUARDED = 1,
+  XRP_PRECISION_FP64 = 2
+} xrp_precision;
+
+/* Number semantics of scene text: Go's encoding/json decodes every number
+ * as float64; yaml.v3 keeps integer literals as int (so `rho: 1` is
+ * rejected where the reference type-asserts float64). */
+typedef enum { XRP_NUMBERS_JSON = 0, XRP_NUMBERS_YAML = 1 } xrp_number_mode;
+
+typedef struct xrp_context xrp_context; /* one device + stream + caches */
+typedef struct xrp_scene xrp_scene;     /* loaded object tree + deformation */
+
+/* Camera position on the sphere of radius R (main.go:282-285). */
+typedef struct {
+  double azimuthal_deg;
+  double polar_deg;
+} xrp_view_angle;
+
+/* Render parameters (main.go:310-330 globals and flags).  Set
+ * struct_size = sizeof(xrp_render_params); fields added later are
+ * appended so older callers keep working. */
+typedef struct {
+  uint32_t struct_size;
+  int32_t resolution;        /* square image side, pixels */
+  double ds;                 /* step (simple) or coarse step DS (hierarchical); > 0 */
+  double R;                  /* camera distance, default 4 */
+  double fov_deg;            /* field of view, default 40 */
+  double flat_field;         /* optical depth added to every ray */
+  double density_multiplier; /* must be non-zero (api.go:101-103) */
+  int32_t integrator;        /* xrp_integrator */
+  int32_t precision;         /* xrp_precision */
+  int32_t audit;             /* 1: also evaluate every sample in FP64 and count
+                                FP32 decisions that disagree (slow; testing) */
+  int32_t reserved;
+} xrp_render_params;
+
+/* Fills p with the reference's CLI defaults (main.go:651-761):
+ * resolution 512, ds -1 (caller must resolve; see xrp_scene_auto_ds),
+ * R 4, fov 40, flat_field 0, density_multiplier 1, hierarchical. */
+void xrp_default_params(xrp_render_params *p);
+
+typedef struct {
+  uint64_t rays;
+  uint64_t samples;        /* density evaluations after bounding-box clipping */
+  uint64_t reevaluated;    /* guard-band samples re-evaluated in FP64 */
+  uint64_t audit_mismatch; /* audit mode: FP32-clear samples that differ in FP64 */
+  uint64_t fine_samples;   /* hierarchical fine sub-step evaluations */
+  double kernel_ms;        /* device time of the render kernels */
+  double position_error;   /* FP32 object-space position error bound used */
+  int32_t precision_used;  /* xrp_precision actually run */
+  int32_t reserved;
+} xrp_render_stats;
+
+int xrp_abi_version(void);
+const char *xrp_last_error(void);
+
+/* ---- context -------------------------------------------------------------- */
+int xrp_context_create(int device, xrp_context **out);
+void xrp_context_destroy(xrp_context *ctx);
+int xrp_context_synchronize(xrp_context *ctx);
+
+/* ---- scenes (host only; no device work) ----------------------------------- */
+/* object_text: an object description (objects.NewObject, objects.go:658-689).
+ * deformation_text: NULL/"" or one deformation (deformations.go:308-343).
+ * Text is JSON; number_mode chooses the decoder semantics.  base_dir
+ * resolves relative voxel_grid raw paths (NULL = current directory). */
+int xrp_scene_create(const char *object_text, const char *deformation_text,
+                     int number_mode, const char *base_dir, xrp_scene **out);
+/* Voxel-grid scene from caller memory: rho[z*nx*ny + x*ny + y] in [0,1]. */
+int xrp_scene_create_voxels(const float *rho, int nx, int ny, int nz,
+                            const char *deformation_text, int number_mode,
+                            xrp_scene **out);
+void xrp_scene_destroy(xrp_scene *scene);
+/* MinFeatureSize of the root object (objects.go MinFeatureSize methods). */
+int xrp_scene_min_feature_size(const xrp_scene *scene, double *out);
+/* ds rule of main.go:350-353: MinFeatureSize / 5. */
+int xrp_scene_auto_ds(const xrp_scene *scene, double *out);
+/* World-space box outside which the scene density is zero (after the
+ * deformation preimage); infinite entries mean unbounded.  *empty = 1 when
+ * the density is zero everywhere. */
+int xrp_scene_clip_box(const xrp_scene *scene, double lo[3], double hi[3],
+                       int *empty);
+/* Number of flattened device nodes and deformation stages. */
+int xrp_scene_layout(const xrp_scene *scene, int *nodes, int *deformations,
+                     int *has_voxels);
+
+/* ---- cameras and sample grids (host only) --------------------------------- */
+/* eye and row-major cam2world = inverse(LookAt(eye, 0, +z)). */
+int xrp_camera_from_angles(double azimuthal_deg, double polar_deg, double R,
+                           double eye[3], double cam2world[16]);
+/* Count of s = smin, smin+ds, ... with s < smax (accumulated in FP64). */
+int xrp_sample_count(double smin, double smax, double ds, int64_t *count);
+
+/* ---- rendering ------------------------------------------------------------ */
+/* out: n_views * res * res float32, [view][i][j], i = image x (host memory). */
+int xrp_render(xrp_context *ctx, xrp_scene *scene, const xrp_render_params *p,
+               const xrp_view_angle *views, int n_views, float *out,
+               xrp_render_stats *stats);
+/* Same, out is device memory on the context's device; returns when done. */
+int xrp_render_device(xrp_context *ctx, xrp_scene *scene,
+                      const xrp_render_params *p, const xrp_view_angle *views,
+                      int n_views, float *out_device, xrp_render_stats *stats);
+
+/* Explicit-camera voxel render replacing RenderVolumeProjectionsCUDA
+ * (cuda_backend.h:70-112) with the CPU renderer's semantics: software
+ * trilinear with the (N-1) convention, FP64 accumulated sample positions.
+ * volume[k*nx*ny + i*ny + j]; cameras row-major cam2world (FP64). */
+typedef struct {
+  double eye[3];
+  double cam2world[16];
+  double fov_deg;
+  double R;
+} xrp_camera;
+int xrp_render_volume_projections(xrp_context *ctx, const float *volume, int nx,
+                                  int ny, int nz, const xrp_camera *cameras,
+                                  int n_cameras, int image_res, double ds,
+                                  double flat_field, float *out_images);
+
+/* ---- reference JSON API (api.go:82-180) ----------------------------------- */
+/* Renders as the reference's RenderProjections does (PNG frames,
+ * transforms.json, object.json) on the GPU.  Returns a malloc'd JSON
+ * RenderResult; release it with FreeString. */
+char *RenderProjections(const char *json_params);
+void FreeString(char *str);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XRP_H */

@@ -1,0 +1,312 @@
+"""ctypes binding of libxrp's C ABI (include/xrp.h).
+
+The shared library is built in-tree by ``xray_projection_render_amd.build``.
+Nothing here falls back to a CPU implementation: when the library or a GPU
+is missing the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from typing import Dict, Iterable, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("XRP_LIB") or os.path.join(HERE, "libxrp.so")
+
+XRP_OK = 0
+STATUS_NAMES = {
+    0: "XRP_OK", 1: "XRP_ERR_NULL", 2: "XRP_ERR_DIMS", 3: "XRP_ERR_DEVICE",
+    4: "XRP_ERR_ALLOC", 10: "XRP_ERR_SCENE", 11: "XRP_ERR_DEFORMATION",
+    12: "XRP_ERR_ARG", 13: "XRP_ERR_UNSUPPORTED", 14: "XRP_ERR_IO",
+}
+INTEGRATORS = {"simple": 0, "hierarchical": 1}
+PRECISIONS = {"auto": 0, "guarded": 1, "fp64": 2}
+NUMBERS = {"json": 0, "yaml": 1}
+
+EXPORTED = [
+    "xrp_abi_version", "xrp_last_error", "xrp_default_params",
+    "xrp_context_create", "xrp_context_destroy", "xrp_context_synchronize",
+    "xrp_scene_create", "xrp_scene_create_voxels", "xrp_scene_destroy",
+    "xrp_scene_min_feature_size", "xrp_scene_auto_ds", "xrp_scene_clip_box",
+    "xrp_scene_layout", "xrp_camera_from_angles", "xrp_sample_count",
+    "xrp_render", "xrp_render_device", "xrp_render_volume_projections",
+]
+
+
+class XrpError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("resolution", ctypes.c_int32),
+        ("ds", ctypes.c_double), ("R", ctypes.c_double),
+        ("fov_deg", ctypes.c_double), ("flat_field", ctypes.c_double),
+        ("density_multiplier", ctypes.c_double),
+        ("integrator", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("audit", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [
+        ("rays", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+        ("reevaluated", ctypes.c_uint64), ("audit_mismatch", ctypes.c_uint64),
+        ("fine_samples", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
+        ("position_error", ctypes.c_double), ("precision_used", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> Dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class ViewAngle(ctypes.Structure):
+    _fields_ = [("azimuthal_deg", ctypes.c_double), ("polar_deg", ctypes.c_double)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_double * 3), ("cam2world", ctypes.c_double * 16),
+                ("fov_deg", ctypes.c_double), ("R", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libxrp.so not found at {LIB_PATH}; build it with "
+            "`python -m xray_projection_render_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    DP = ctypes.POINTER(ctypes.c_double)
+    FP = ctypes.POINTER(ctypes.c_float)
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "xrp_abi_version": (I, []),
+        "xrp_last_error": (ctypes.c_char_p, []),
+        "xrp_default_params": (None, [ctypes.POINTER(RenderParams)]),
+        "xrp_context_create": (I, [I, PP]),
+        "xrp_context_destroy": (None, [P]),
+        "xrp_context_synchronize": (I, [P]),
+        "xrp_scene_create": (I, [ctypes.c_char_p, ctypes.c_char_p, I, ctypes.c_char_p, PP]),
+        "xrp_scene_create_voxels": (I, [FP, I, I, I, ctypes.c_char_p, I, PP]),
+        "xrp_scene_destroy": (None, [P]),
+        "xrp_scene_min_feature_size": (I, [P, DP]),
+        "xrp_scene_auto_ds": (I, [P, DP]),
+        "xrp_scene_clip_box": (I, [P, DP, DP, ctypes.POINTER(I)]),
+        "xrp_scene_layout": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "xrp_camera_from_angles": (I, [D, D, D, DP, DP]),
+        "xrp_sample_count": (I, [D, D, D, ctypes.POINTER(ctypes.c_int64)]),
+        "xrp_render": (I, [P, P, ctypes.POINTER(RenderParams), ctypes.POINTER(ViewAngle),
+                           I, FP, ctypes.POINTER(RenderStats)]),
+        "xrp_render_device": (I, [P, P, ctypes.POINTER(RenderParams),
+                                  ctypes.POINTER(ViewAngle), I, P,
+                                  ctypes.POINTER(RenderStats)]),
+        "xrp_render_volume_projections": (I, [P, FP, I, I, I, ctypes.POINTER(Camera), I, I,
+                                              D, D, FP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code: int) -> None:
+    if code != XRP_OK:
+        msg = lib().xrp_last_error()
+        raise XrpError(code, msg.decode() if msg else "")
+
+
+def _dp(arr: np.ndarray):
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def camera_from_angles(az: float, polar: float, R: float = 4.0) -> Tuple[np.ndarray, np.ndarray]:
+    eye = np.zeros(3)
+    cam = np.zeros(16)
+    check(lib().xrp_camera_from_angles(az, polar, R, _dp(eye), _dp(cam)))
+    return eye, cam.reshape(4, 4)
+
+
+def sample_count(smin: float, smax: float, ds: float) -> int:
+    n = ctypes.c_int64()
+    check(lib().xrp_sample_count(smin, smax, ds, ctypes.byref(n)))
+    return n.value
+
+
+class Context:
+    """One device, one HIP stream and the cached device buffers."""
+
+    def __init__(self, device: int = 0):
+        p = ctypes.c_void_p()
+        check(lib().xrp_context_create(device, ctypes.byref(p)))
+        self.ptr = p
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().xrp_context_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """An object tree (+ optional deformation) parsed by libxrp."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    @classmethod
+    def from_text(cls, object_text: str, deformation_text: Optional[str] = None,
+                  numbers: str = "json", base_dir: Optional[str] = None) -> "Scene":
+        p = ctypes.c_void_p()
+        check(lib().xrp_scene_create(
+            object_text.encode(), deformation_text.encode() if deformation_text else None,
+            NUMBERS[numbers], base_dir.encode() if base_dir else None, ctypes.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def from_dict(cls, obj: Dict, deformation: Optional[Dict] = None,
+                  numbers: str = "yaml", base_dir: Optional[str] = None) -> "Scene":
+        """Dicts keep Python int/float, so YAML-decoded data passes through
+        with yaml.v3 number semantics (numbers="yaml")."""
+        return cls.from_text(json.dumps(obj), json.dumps(deformation) if deformation else None,
+                             numbers, base_dir)
+
+    @classmethod
+    def from_voxels(cls, rho: np.ndarray, deformation: Optional[Dict] = None,
+                    numbers: str = "yaml") -> "Scene":
+        """rho shaped (nz, nx, ny): flat index z*nx*ny + x*ny + y."""
+        arr = np.ascontiguousarray(rho, dtype=np.float32)
+        nz, nx, ny = arr.shape
+        p = ctypes.c_void_p()
+        check(lib().xrp_scene_create_voxels(
+            arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nx, ny, nz,
+            json.dumps(deformation).encode() if deformation else None,
+            NUMBERS[numbers], ctypes.byref(p)))
+        return cls(p)
+
+    def min_feature_size(self) -> float:
+        v = ctypes.c_double()
+        check(lib().xrp_scene_min_feature_size(self.ptr, ctypes.byref(v)))
+        return v.value
+
+    def auto_ds(self) -> float:
+        v = ctypes.c_double()
+        check(lib().xrp_scene_auto_ds(self.ptr, ctypes.byref(v)))
+        return v.value
+
+    def clip_box(self):
+        lo, hi = np.zeros(3), np.zeros(3)
+        e = ctypes.c_int()
+        check(lib().xrp_scene_clip_box(self.ptr, _dp(lo), _dp(hi), ctypes.byref(e)))
+        return lo, hi, bool(e.value)
+
+    def layout(self):
+        n, d, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().xrp_scene_layout(self.ptr, ctypes.byref(n), ctypes.byref(d),
+                                     ctypes.byref(v)))
+        return n.value, d.value, bool(v.value)
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().xrp_scene_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_params(resolution: int, ds: float, R: float = 4.0, fov: float = 40.0,
+                flat_field: float = 0.0, density_multiplier: float = 1.0,
+                integrator: str = "simple", precision: str = "auto",
+                audit: bool = False) -> RenderParams:
+    p = RenderParams()
+    lib().xrp_default_params(ctypes.byref(p))
+    p.resolution = resolution
+    p.ds = ds
+    p.R = R
+    p.fov_deg = fov
+    p.flat_field = flat_field
+    p.density_multiplier = density_multiplier
+    p.integrator = INTEGRATORS[integrator]
+    p.precision = PRECISIONS[precision]
+    p.audit = 1 if audit else 0
+    return p
+
+
+def _views(angles: Sequence[Tuple[float, float]]):
+    arr = (ViewAngle * len(angles))()
+    for k, (az, po) in enumerate(angles):
+        arr[k].azimuthal_deg = az
+        arr[k].polar_deg = po
+    return arr
+
+
+def render(ctx: Context, scene: Scene, angles: Sequence[Tuple[float, float]],
+           resolution: int, ds: float, **kw) -> Tuple[np.ndarray, Dict]:
+    """Transmittance images [view][i][j] (float32) into host memory."""
+    p = make_params(resolution, ds, **kw)
+    out = np.empty((len(angles), resolution, resolution), dtype=np.float32)
+    st = RenderStats()
+    check(lib().xrp_render(ctx.ptr, scene.ptr, ctypes.byref(p), _views(angles),
+                           len(angles), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                           ctypes.byref(st)))
+    return out, st.as_dict()
+
+
+def render_device(ctx: Context, scene: Scene, angles: Sequence[Tuple[float, float]],
+                  resolution: int, ds: float, out_ptr: int, **kw) -> Dict:
+    """Render into device memory at out_ptr (e.g. a torch tensor's data_ptr())."""
+    p = make_params(resolution, ds, **kw)
+    st = RenderStats()
+    check(lib().xrp_render_device(ctx.ptr, scene.ptr, ctypes.byref(p), _views(angles),
+                                  len(angles), ctypes.c_void_p(out_ptr), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def render_volume_projections(ctx: Context, volume: np.ndarray,
+                              cameras: Iterable[Tuple[np.ndarray, np.ndarray, float, float]],
+                              image_res: int, ds: float, flat_field: float = 0.0) -> np.ndarray:
+    """volume shaped (nz, nx, ny); cameras = (eye, cam2world 4x4, fov_deg, R)."""
+    vol = np.ascontiguousarray(volume, dtype=np.float32)
+    nz, nx, ny = vol.shape
+    cams = list(cameras)
+    arr = (Camera * len(cams))()
+    for k, (eye, cam, fov, R) in enumerate(cams):
+        arr[k].eye[:] = [float(v) for v in eye]
+        arr[k].cam2world[:] = [float(v) for v in np.asarray(cam).reshape(-1)]
+        arr[k].fov_deg = fov
+        arr[k].R = R
+    out = np.empty((len(cams), image_res, image_res), dtype=np.float32)
+    check(lib().xrp_render_volume_projections(
+        ctx.ptr, vol.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nx, ny, nz, arr,
+        len(cams), image_res, ds, flat_field,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out

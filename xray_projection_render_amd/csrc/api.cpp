@@ -1,0 +1,627 @@
+// api.cpp - C ABI of libxrp (declared in include/xrp.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/xrp.h"
+#include "camera.hpp"
+#include "json.hpp"
+#include "march.hpp"
+#include "scene.hpp"
+
+namespace {
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+std::mutex g_registry_mu;  // scene uploads <-> contexts
+}  // namespace
+
+struct xrp_scene;
+
+struct xrp_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::mutex mu;
+  DevBuf views, table, stats, out, nodes, defs;
+  // sample table cache
+  int table_mode = -1;
+  double table_smin = NAN, table_smax = NAN, table_ds = NAN;
+  std::vector<double> table_host;
+  std::set<xrp_scene*> scenes;  // scenes holding device copies here
+};
+
+struct xrp_scene {
+  std::unique_ptr<xrp::Scene> s;
+  std::map<xrp_context*, std::vector<void*>> uploads;  // voxel grids per context
+};
+
+#define HIPCHK(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(XRP_ERR_DEVICE, std::string(#expr) + " failed: " +        \
+                                      hipGetErrorString(e_));               \
+  } while (0)
+
+static void free_uploads(xrp_scene* sc, xrp_context* ctx) {
+  auto it = sc->uploads.find(ctx);
+  if (it == sc->uploads.end()) return;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(ctx->device);
+  for (void* p : it->second) (void)hipFree(p);
+  (void)hipSetDevice(prev);
+  sc->uploads.erase(it);
+}
+
+extern "C" {
+
+int xrp_abi_version(void) { return XRP_ABI_VERSION; }
+
+const char* xrp_last_error(void) { return g_error.c_str(); }
+
+void xrp_default_params(xrp_render_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->struct_size = sizeof *p;
+  p->resolution = 512;
+  p->ds = -1.0;
+  p->R = 4.0;
+  p->fov_deg = 40.0;
+  p->flat_field = 0.0;
+  p->density_multiplier = 1.0;
+  p->integrator = XRP_INTEGRATE_HIERARCHICAL;
+  p->precision = XRP_PRECISION_AUTO;
+}
+
+int xrp_context_create(int device, xrp_context** out) {
+  if (!out) return fail(XRP_ERR_NULL, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(XRP_ERR_ARG, "device " + std::to_string(device) +
+                                 " out of range (" + std::to_string(n) +
+                                 " devices)");
+  HIPCHK(hipSetDevice(device));
+  auto* c = new xrp_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(XRP_ERR_DEVICE, std::string("context set-up failed: ") +
+                                    hipGetErrorString(e));
+  }
+  *out = c;
+  return XRP_OK;
+}
+
+void xrp_context_destroy(xrp_context* ctx) {
+  if (!ctx) return;
+  {
+    std::lock_guard<std::mutex> g(g_registry_mu);
+    for (xrp_scene* sc : ctx->scenes) free_uploads(sc, ctx);
+    ctx->scenes.clear();
+  }
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (DevBuf* b : {&ctx->views, &ctx->table, &ctx->stats, &ctx->out,
+                    &ctx->nodes, &ctx->defs})
+    b->release();
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int xrp_context_synchronize(xrp_context* ctx) {
+  if (!ctx) return fail(XRP_ERR_NULL, "ctx is NULL");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return XRP_OK;
+}
+
+static int parse_text(const char* text, xrp::json::Value& out, bool& present) {
+  present = text && text[0];
+  if (!present) return XRP_OK;
+  try {
+    out = xrp::json::parse(text);
+  } catch (const std::exception& e) {
+    return fail(XRP_ERR_ARG, e.what());
+  }
+  return XRP_OK;
+}
+
+int xrp_scene_create(const char* object_text, const char* deformation_text,
+                     int number_mode, const char* base_dir, xrp_scene** out) {
+  if (!out) return fail(XRP_ERR_NULL, "out is NULL");
+  *out = nullptr;
+  if (!object_text) return fail(XRP_ERR_NULL, "object_text is NULL");
+  if (number_mode != XRP_NUMBERS_JSON && number_mode != XRP_NUMBERS_YAML)
+    return fail(XRP_ERR_ARG, "unknown number_mode");
+  xrp::json::Value obj, def;
+  bool has_obj, has_def;
+  int rc = parse_text(object_text, obj, has_obj);
+  if (rc) return rc;
+  if (!has_obj) return fail(XRP_ERR_SCENE, "empty object description");
+  rc = parse_text(deformation_text, def, has_def);
+  if (rc) return rc;
+  auto* sc = new xrp_scene();
+  try {
+    sc->s = xrp::load_scene(obj, nullptr, (xrp::NumberMode)number_mode,
+                            base_dir ? base_dir : "");
+  } catch (const std::exception& e) {
+    delete sc;
+    return fail(XRP_ERR_SCENE, e.what());
+  }
+  if (has_def) {
+    try {
+      xrp::append_deformation(def, (xrp::NumberMode)number_mode, sc->s->defs);
+      xrp::finalize_scene(*sc->s);
+    } catch (const std::exception& e) {
+      delete sc;
+      return fail(XRP_ERR_DEFORMATION, e.what());
+    }
+  }
+  *out = sc;
+  return XRP_OK;
+}
+
+int xrp_scene_create_voxels(const float* rho, int nx, int ny, int nz,
+                            const char* deformation_text, int number_mode,
+                            xrp_scene** out) {
+  if (!out) return fail(XRP_ERR_NULL, "out is NULL");
+  *out = nullptr;
+  if (!rho) return fail(XRP_ERR_NULL, "rho is NULL");
+  if (nx <= 0 || ny <= 0 || nz <= 0)
+    return fail(XRP_ERR_DIMS, "voxel grid dimensions must be positive");
+  xrp::json::Value def;
+  bool has_def;
+  int rc = parse_text(deformation_text, def, has_def);
+  if (rc) return rc;
+  auto* sc = new xrp_scene();
+  try {
+    sc->s = xrp::voxel_scene(rho, nx, ny, nz, has_def ? &def : nullptr,
+                             (xrp::NumberMode)number_mode);
+  } catch (const std::exception& e) {
+    delete sc;
+    return fail(XRP_ERR_DEFORMATION, e.what());
+  }
+  *out = sc;
+  return XRP_OK;
+}
+
+void xrp_scene_destroy(xrp_scene* sc) {
+  if (!sc) return;
+  {
+    std::lock_guard<std::mutex> g(g_registry_mu);
+    std::vector<xrp_context*> ctxs;
+    for (auto& kv : sc->uploads) ctxs.push_back(kv.first);
+    for (xrp_context* c : ctxs) {
+      free_uploads(sc, c);
+      c->scenes.erase(sc);
+    }
+  }
+  delete sc;
+}
+
+int xrp_scene_min_feature_size(const xrp_scene* sc, double* out) {
+  if (!sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
+  *out = sc->s->min_feature;
+  return XRP_OK;
+}
+
+int xrp_scene_auto_ds(const xrp_scene* sc, double* out) {
+  if (!sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
+  *out = sc->s->min_feature / 5.0;  // main.go:350-353
+  return XRP_OK;
+}
+
+int xrp_scene_clip_box(const xrp_scene* sc, double lo[3], double hi[3],
+                       int* empty) {
+  if (!sc || !lo || !hi || !empty) return fail(XRP_ERR_NULL, "NULL argument");
+  const auto& b = sc->s->world_box;
+  *empty = b.empty() ? 1 : 0;
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = b.lo[k];
+    hi[k] = b.hi[k];
+  }
+  return XRP_OK;
+}
+
+int xrp_scene_layout(const xrp_scene* sc, int* nodes, int* deformations,
+                     int* has_voxels) {
+  if (!sc) return fail(XRP_ERR_NULL, "scene is NULL");
+  if (nodes) *nodes = (int)sc->s->nodes.size();
+  if (deformations) *deformations = (int)sc->s->ddefs.size();
+  if (has_voxels) *has_voxels = sc->s->has_voxels ? 1 : 0;
+  return XRP_OK;
+}
+
+int xrp_camera_from_angles(double az, double polar, double R, double eye[3],
+                           double cam[16]) {
+  if (!eye || !cam) return fail(XRP_ERR_NULL, "NULL argument");
+  xrp::camera_from_angles(az, polar, R, eye, cam);
+  return XRP_OK;
+}
+
+int xrp_sample_count(double smin, double smax, double ds, int64_t* count) {
+  if (!count) return fail(XRP_ERR_NULL, "count is NULL");
+  if (!(ds > 0.0) || !std::isfinite(ds) || !std::isfinite(smin) ||
+      !std::isfinite(smax))
+    return fail(XRP_ERR_ARG, "ds must be positive and bounds finite");
+  int64_t n = 0;
+  for (double s = smin; s < smax; s += ds) ++n;
+  *count = n;
+  return XRP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Render core.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct RenderJob {
+  std::vector<xrp::DevView> views;
+  int res;
+  double ds, R, fov, flat_field, dm;
+  int integrator, precision, audit;
+};
+
+int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs) {
+  std::lock_guard<std::mutex> g(g_registry_mu);
+  auto it = sc->uploads.find(ctx);
+  if (it != sc->uploads.end()) {
+    ptrs = it->second;
+    return XRP_OK;
+  }
+  std::vector<const xrp::HNode*> vox;
+  // preorder, matching the flattened node order
+  std::function<void(const xrp::HNode&)> walk = [&](const xrp::HNode& n) {
+    if (n.kind == xrp::K_VOXEL) vox.push_back(&n);
+    for (auto& k : n.kids) walk(k);
+  };
+  walk(sc->s->root);
+  std::vector<void*> out;
+  for (auto* n : vox) {
+    void* p = nullptr;
+    size_t bytes = n->vox->size() * sizeof(float);
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 4));
+    if (e == hipSuccess)
+      e = hipMemcpy(p, n->vox->data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (p) (void)hipFree(p);
+      for (void* q : out) (void)hipFree(q);
+      return fail(XRP_ERR_ALLOC, std::string("voxel upload failed: ") +
+                                     hipGetErrorString(e));
+    }
+    out.push_back(p);
+  }
+  sc->uploads[ctx] = out;
+  ctx->scenes.insert(sc);
+  ptrs = out;
+  return XRP_OK;
+}
+
+int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
+                float* out_dev, xrp_render_stats* st) {
+  using namespace xrp;
+  Scene& s = *sc->s;
+  const int nv = (int)job.views.size();
+  const int res = job.res;
+  HIPCHK(hipSetDevice(ctx->device));
+
+  int precision = job.precision;
+  if (precision == XRP_PRECISION_AUTO)
+    precision = s.has_voxels ? XRP_PRECISION_FP64 : XRP_PRECISION_GUARDED;
+
+  // Sample table (cached per context).
+  const double smin = job.R - kCubeHalfDiagonal, smax = job.R + kCubeHalfDiagonal;
+  if (ctx->table_mode != job.integrator || ctx->table_smin != smin ||
+      ctx->table_smax != smax || ctx->table_ds != job.ds) {
+    ctx->table_host = job.integrator == XRP_INTEGRATE_HIERARCHICAL
+                          ? coarse_samples(smin, smax, job.ds)
+                          : simple_samples(smin, smax, job.ds);
+    HIPCHK(ctx->table.ensure(std::max<size_t>(1, ctx->table_host.size()) *
+                             sizeof(double)));
+    if (!ctx->table_host.empty())
+      HIPCHK(hipMemcpy(ctx->table.p, ctx->table_host.data(),
+                       ctx->table_host.size() * sizeof(double),
+                       hipMemcpyHostToDevice));
+    ctx->table_mode = job.integrator;
+    ctx->table_smin = smin;
+    ctx->table_smax = smax;
+    ctx->table_ds = job.ds;
+  }
+  const int K = (int)ctx->table_host.size();
+
+  // Clip box and FP32 error model.
+  const Box3& wb = s.world_box;
+  int clip = 1;
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = wb.lo[k];
+    hi[k] = wb.hi[k];
+  }
+  bool all_inf = true;
+  for (int k = 0; k < 3; ++k)
+    if (std::isfinite(lo[k]) || std::isfinite(hi[k])) all_inf = false;
+  if (wb.empty())
+    clip = 2;
+  else if (all_inf)
+    clip = 0;
+  double Bw = 0;
+  if (clip == 1 && wb.finite()) {
+    for (int k = 0; k < 3; ++k)
+      Bw = std::max({Bw, std::fabs(lo[k]), std::fabs(hi[k])});
+    Bw += 3.0 * job.ds;
+  } else {
+    Bw = 2.0 * std::fabs(job.R) + kCubeHalfDiagonal + 1.0;
+  }
+  double delta = 0.0;
+  if (precision == XRP_PRECISION_GUARDED) {
+    double step = job.integrator == XRP_INTEGRATE_SIMPLE ? 32.0 * job.ds : 0.0;
+    delta = compute_guards(s, Bw, step);
+    // Tiny features against a coarse FP32 grid: fall back to FP64.
+    if (!(delta < 1e-3 * s.min_feature)) precision = XRP_PRECISION_FP64;
+  }
+
+  std::vector<void*> vptr;
+  if (s.has_voxels) {
+    int rc = upload_voxels(ctx, sc, vptr);
+    if (rc) return rc;
+  }
+  size_t vi = 0;
+  for (auto& n : s.nodes)
+    if (n.kind == K_VOXEL) n.vox = (const float*)vptr[vi++];
+
+  HIPCHK(ctx->nodes.ensure(s.nodes.size() * sizeof(DevNode)));
+  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, s.ddefs.size()) * sizeof(DevDeform)));
+  HIPCHK(ctx->views.ensure(nv * sizeof(DevView)));
+  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
+  HIPCHK(hipMemcpyAsync(ctx->nodes.p, s.nodes.data(),
+                        s.nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice,
+                        ctx->stream));
+  if (!s.ddefs.empty())
+    HIPCHK(hipMemcpyAsync(ctx->defs.p, s.ddefs.data(),
+                          s.ddefs.size() * sizeof(DevDeform),
+                          hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->views.p, job.views.data(), nv * sizeof(DevView),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long),
+                        ctx->stream));
+
+  MarchParams mp;
+  std::memset(&mp, 0, sizeof mp);
+  mp.nodes = (const DevNode*)ctx->nodes.p;
+  mp.defs = (const DevDeform*)ctx->defs.p;
+  mp.views = (const DevView*)ctx->views.p;
+  mp.s_table = (const double*)ctx->table.p;
+  mp.stats = (unsigned long long*)ctx->stats.p;
+  mp.n_defs = (int)s.ddefs.size();
+  mp.res = res;
+  mp.K = K;
+  mp.focal = focal_length(job.fov);
+  mp.ds = job.ds;
+  mp.ds_fine = job.ds / 10.0;
+  mp.smin = smin;
+  mp.smax = smax;
+  mp.flat_field = job.flat_field;
+  mp.dm = job.dm;
+  for (int k = 0; k < 3; ++k) {
+    mp.lo[k] = lo[k];
+    mp.hi[k] = hi[k];
+  }
+  mp.clip = clip == 1 ? 1 : 0;
+  if (clip == 2) mp.K = 0;  // zero density everywhere: T = flat_field
+  mp.audit = job.audit;
+  mp.out_view_stride = res * res;
+
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  const int kMaxGridY = 65535;
+  for (int v0 = 0; v0 < nv; v0 += kMaxGridY) {
+    mp.view_offset = v0;
+    mp.n_views = std::min(kMaxGridY, nv - v0);
+    mp.out = out_dev + (size_t)v0 * res * res;
+    int prec = precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED;
+    int integ = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
+                                                             : INT_SIMPLE;
+    int e = launch_march(mp, prec, integ, ctx->stream);
+    if (e != 0)
+      return fail(XRP_ERR_DEVICE, std::string("kernel launch failed: ") +
+                                      hipGetErrorString((hipError_t)e));
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  unsigned long long hst[ST_COUNT];
+  HIPCHK(hipMemcpyAsync(hst, ctx->stats.p, sizeof hst, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (st) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::memset(st, 0, sizeof *st);
+    st->rays = (uint64_t)nv * res * res;
+    st->samples = hst[ST_SAMPLES];
+    st->reevaluated = hst[ST_REEVAL];
+    st->audit_mismatch = hst[ST_AUDIT_MISS];
+    st->fine_samples = hst[ST_FINE];
+    st->kernel_ms = ms;
+    st->position_error = delta;
+    st->precision_used = precision;
+  }
+  return XRP_OK;
+}
+
+int check_params(const xrp_render_params* p, int n_views) {
+  if (!p) return fail(XRP_ERR_NULL, "params is NULL");
+  if (p->struct_size < sizeof(xrp_render_params))
+    return fail(XRP_ERR_ARG, "params.struct_size too small");
+  if (p->resolution <= 0 || n_views <= 0)
+    return fail(XRP_ERR_DIMS, "resolution and view count must be positive");
+  if (p->resolution > 32768)
+    return fail(XRP_ERR_DIMS, "resolution above 32768");
+  if (!(p->ds > 0.0) || !std::isfinite(p->ds))
+    return fail(XRP_ERR_ARG,
+                "ds must be positive (resolve ds < 0 with xrp_scene_auto_ds)");
+  if (p->density_multiplier == 0.0)
+    return fail(XRP_ERR_ARG,
+                "density_multiplier is 0; all densities will be zero and the "
+                "render will produce a blank image");
+  if (!std::isfinite(p->R) || !std::isfinite(p->fov_deg) ||
+      !std::isfinite(p->flat_field) || !std::isfinite(p->density_multiplier))
+    return fail(XRP_ERR_ARG, "R, fov, flat_field and density_multiplier must be finite");
+  if (p->integrator != XRP_INTEGRATE_SIMPLE &&
+      p->integrator != XRP_INTEGRATE_HIERARCHICAL)
+    return fail(XRP_ERR_ARG, "unknown integrator");
+  if (p->precision < XRP_PRECISION_AUTO || p->precision > XRP_PRECISION_FP64)
+    return fail(XRP_ERR_ARG, "unknown precision");
+  double K = 2 * xrp::kCubeHalfDiagonal / p->ds;
+  if (K > 5e7) return fail(XRP_ERR_ARG, "ds too small: more than 5e7 samples per ray");
+  return XRP_OK;
+}
+
+RenderJob make_job(const xrp_render_params* p, const xrp_view_angle* views,
+                   int n_views) {
+  RenderJob job;
+  job.res = p->resolution;
+  job.ds = p->ds;
+  job.R = p->R;
+  job.fov = p->fov_deg;
+  job.flat_field = p->flat_field;
+  job.dm = p->density_multiplier;
+  job.integrator = p->integrator;
+  job.precision = p->precision;
+  job.audit = p->audit;
+  job.views.resize(n_views);
+  for (int v = 0; v < n_views; ++v) {
+    xrp::DevView& dv = job.views[v];
+    std::memset(&dv, 0, sizeof dv);
+    xrp::camera_from_angles(views[v].azimuthal_deg, views[v].polar_deg, p->R,
+                            dv.eye, dv.cam);
+  }
+  return job;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xrp_render_device(xrp_context* ctx, xrp_scene* sc,
+                      const xrp_render_params* p, const xrp_view_angle* views,
+                      int n_views, float* out_device, xrp_render_stats* st) {
+  if (!ctx || !sc || !views || !out_device)
+    return fail(XRP_ERR_NULL, "NULL argument");
+  int rc = check_params(p, n_views);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  RenderJob job = make_job(p, views, n_views);
+  return render_core(ctx, sc, job, out_device, st);
+}
+
+int xrp_render(xrp_context* ctx, xrp_scene* sc, const xrp_render_params* p,
+               const xrp_view_angle* views, int n_views, float* out,
+               xrp_render_stats* st) {
+  if (!ctx || !sc || !views || !out) return fail(XRP_ERR_NULL, "NULL argument");
+  int rc = check_params(p, n_views);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t bytes = (size_t)n_views * p->resolution * p->resolution * sizeof(float);
+  HIPCHK(ctx->out.ensure(bytes));
+  RenderJob job = make_job(p, views, n_views);
+  rc = render_core(ctx, sc, job, (float*)ctx->out.p, st);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(out, ctx->out.p, bytes, hipMemcpyDeviceToHost));
+  return XRP_OK;
+}
+
+int xrp_render_volume_projections(xrp_context* ctx, const float* volume, int nx,
+                                  int ny, int nz, const xrp_camera* cams,
+                                  int n_cams, int image_res, double ds,
+                                  double flat_field, float* out) {
+  // cuda_backend.cu:95-100 argument checks, same codes.
+  if (!ctx || !volume || !cams || !out) return fail(XRP_ERR_NULL, "NULL argument");
+  if (nx <= 0 || ny <= 0 || nz <= 0 || image_res <= 0 || n_cams <= 0)
+    return fail(XRP_ERR_DIMS, "dimensions must be positive");
+  if (!(ds > 0.0) || !std::isfinite(ds)) return fail(XRP_ERR_ARG, "ds must be positive");
+  xrp_scene* sc = nullptr;
+  int rc = xrp_scene_create_voxels(volume, nx, ny, nz, nullptr, XRP_NUMBERS_JSON, &sc);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  rc = XRP_OK;
+  size_t img = (size_t)image_res * image_res;
+  hipError_t he = hipSetDevice(ctx->device);
+  if (he == hipSuccess) he = ctx->out.ensure(img * n_cams * sizeof(float));
+  if (he != hipSuccess) rc = fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+  // Consecutive cameras sharing (fov, R) go into one launch.
+  for (int c0 = 0; c0 < n_cams && rc == XRP_OK;) {
+    int c1 = c0 + 1;
+    while (c1 < n_cams && cams[c1].fov_deg == cams[c0].fov_deg &&
+           cams[c1].R == cams[c0].R)
+      ++c1;
+    RenderJob job;
+    job.res = image_res;
+    job.ds = ds;
+    job.R = cams[c0].R;
+    job.fov = cams[c0].fov_deg;
+    job.flat_field = flat_field;
+    job.dm = 1.0;
+    job.integrator = XRP_INTEGRATE_SIMPLE;
+    job.precision = XRP_PRECISION_FP64;
+    job.audit = 0;
+    for (int c = c0; c < c1; ++c) {
+      xrp::DevView dv;
+      std::memset(&dv, 0, sizeof dv);
+      std::memcpy(dv.eye, cams[c].eye, sizeof dv.eye);
+      std::memcpy(dv.cam, cams[c].cam2world, sizeof dv.cam);
+      job.views.push_back(dv);
+    }
+    rc = render_core(ctx, sc, job, (float*)ctx->out.p + img * c0, nullptr);
+    c0 = c1;
+  }
+  if (rc == XRP_OK) {
+    he = hipMemcpy(out, ctx->out.p, img * n_cams * sizeof(float), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) rc = fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+  }
+  xrp_scene_destroy(sc);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// march.hpp - launch interface of the projection kernels (host side).
+#pragma once
+#include <stdint.h>
+
+#include "dev_types.hpp"
+
+namespace xrp {
+
+// One camera: eye and the row-major cam2world matrix (main.go:445-453).
+struct DevView {
+  double eye[3];
+  double cam[16];
+  double pad_;
+};
+
+enum Integrator : int32_t { INT_SIMPLE = 0, INT_HIERARCHICAL = 1 };
+enum Precision : int32_t { PREC_GUARDED = 0, PREC_FP64 = 1 };
+
+// Stat counters accumulated by the kernels (device, uint64 each).
+enum StatSlot {
+  ST_SAMPLES = 0,      // density evaluations made (after clipping)
+  ST_REEVAL = 1,       // guard-band samples re-evaluated in FP64
+  ST_AUDIT_MISS = 2,   // audit: FP32-clear samples whose FP64 value differs
+  ST_FINE = 3,         // hierarchical fine-step evaluations
+  ST_COUNT = 8
+};
+
+struct MarchParams {
+  const DevNode* nodes;
+  const DevDeform* defs;
+  const DevView* views;
+  const double* s_table;   // simple: s_k; hierarchical: coarse right_n
+  float* out;              // [view][i][j] f32 transmittance
+  unsigned long long* stats;
+  int32_t n_defs;
+  int32_t n_views;
+  int32_t res;
+  int32_t K;               // entries in s_table
+  double focal;            // 1/tan(fov/2)
+  double ds;               // simple: ds; hierarchical: DS (coarse)
+  double ds_fine;          // hierarchical: DS/10
+  double smin, smax;
+  double flat_field;
+  double dm;               // density multiplier
+  double lo[3], hi[3];     // world clip box (inflated); infinite = no clip
+  int32_t clip;
+  int32_t audit;           // evaluate every sample in FP64 as well
+  int32_t view_offset;     // first view index of this launch
+  int32_t out_view_stride; // floats between consecutive views in out
+};
+
+// Launch on the given stream; returns hipError_t as int.
+int launch_march(const MarchParams& p, int precision, int integrator,
+                 void* stream);
+
+}  // namespace xrp
