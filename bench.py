@@ -1,0 +1,216 @@
+"""Benchmark: 512^2 projections/s on one MI355X.
+
+Default workload (one step = one render of all views, inputs resident):
+  voxel grid 256^3 float32 (64 seeded Gaussian blobs, world [-0.5,0.5]^3),
+  512x512 pixels, 64 views at random azimuth / polar 90, simple integrator,
+  ds = 2/256/5.  Other workloads: --workload csg|lattice|gyroid.
+
+Prints one JSON line with the throughput, the dominant kernel's roofline
+position (HIP-event kernel time, algorithmic bytes or flops per sample)
+and the FP64 CPU oracle timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import xray_projection_render_amd as X  # noqa: E402
+from xray_projection_render_amd import workloads as W  # noqa: E402
+
+METRIC = "512² projections/s & ray-samples/s; plain vs surrogate at ≤1e-3 error"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic cost per in-box sample (cost table of the benchmark definition:
+# FMA = 2 flops, add/mul/div/sqrt = 1, compares free).
+FLOPS_PER_SAMPLE = {"csg": 42.0, "voxel": 54.0, "lattice": 196.0, "gyroid": 35.0}
+BYTES_PER_SAMPLE = {"voxel": 32.0}  # 8 float32 corner gathers
+
+
+def make_workload(name: str) -> W.Workload:
+    if name == "voxel":
+        return W.voxel()
+    if name == "csg":
+        return W.csg()
+    if name == "lattice":
+        return W.lattice()
+    if name == "gyroid":
+        return W.gyroid_lattice(resolution=512, n_views=16)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def inbox_samples(w: W.Workload, lo: np.ndarray, hi: np.ndarray) -> int:
+    """Count of the reference's sample points s_k (FP64 accumulated) that
+    fall inside the scene's clip box, over every ray of the workload."""
+    smin, smax = w.R - 1.74, w.R + 1.74
+    s = []
+    x = smin
+    while x < smax:
+        s.append(x)
+        x += w.ds
+    s = np.asarray(s)
+    res = w.resolution
+    half = res / 2.0
+    f = 1.0 / math.tan(w.fov / 2 * math.pi / 180)
+    ii, jj = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    v = np.stack([ii / half - 1.0, jj / half - 1.0, np.full(ii.shape, -f),
+                  np.ones(ii.shape)], -1).reshape(-1, 4)
+    total = 0
+    for az, po in w.angles:
+        eye, cam = X.camera_from_angles(az, po, w.R)
+        t = v @ cam.T
+        p = t[:, :3] / t[:, 3:4]
+        d = p - eye
+        d = d / np.linalg.norm(d, axis=1, keepdims=True)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t1 = (lo - eye) / d
+            t2 = (hi - eye) / d
+        tn = np.nanmax(np.minimum(t1, t2), axis=1)
+        tf = np.nanmin(np.maximum(t1, t2), axis=1)
+        ok = tn <= tf
+        a = np.searchsorted(s, tn[ok], side="left")
+        b = np.searchsorted(s, tf[ok], side="right")
+        total += int(np.sum(b - a))
+    return total
+
+
+def cpu_baseline(w: W.Workload, budget_s: float = 12.0) -> dict:
+    """FP64 oracle (all host cores, dynamic row scheduling) on a bounded
+    sample of the workload: whole views in order until ~budget_s of work
+    (first a centre row band of view 0 to size the work)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if w.voxels is not None:
+        nz, nx, ny = w.voxels.shape
+        node = O.voxel_node(w.voxels.astype(np.float64).reshape(-1), nx, ny, nz)
+    else:
+        node = O.build_object(w.obj)
+    d = O.build_deformation(w.deformation) if w.deformation else None
+    sc = O.Scene(node, d)
+    cores = os.cpu_count() or 1
+    res = w.resolution
+    az, po = w.angles[0]
+    band = max(1, res // 32)
+    t = time.perf_counter()
+    sc.render([az], [po], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
+              nthreads=cores, rows=(res // 2 - band // 2, res // 2 - band // 2 + band))
+    probe = time.perf_counter() - t
+    if probe * res / band > budget_s:  # even one view is too much: centre rows
+        rows = max(1, int(band * budget_s / max(probe, 1e-9)))
+        r0 = max(0, res // 2 - rows // 2)
+        rows = min(rows, res - r0)
+        t = time.perf_counter()
+        sc.render([az], [po], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
+                  nthreads=cores, rows=(r0, r0 + rows))
+        spent = time.perf_counter() - t
+        pix, what = rows * res, f"view 0 rows {r0}..{r0 + rows - 1} of {res}"
+    else:
+        views, spent = 0, 0.0
+        while spent < budget_s and views < len(w.angles):
+            a, p = w.angles[views]
+            t = time.perf_counter()
+            sc.render([a], [p], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
+                      nthreads=cores)
+            spent += time.perf_counter() - t
+            views += 1
+        pix, what = views * res * res, f"views 0..{views - 1} of {len(w.angles)}, full images"
+    return {"value": pix / spent / (512 * 512), "unit": "512² projections/s",
+            "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (oracle/xrp_oracle.c, full [R-1.74, R+1.74] segment, "
+                      f"no clipping), {w.name}: {what}, {spent:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="voxel")
+    ap.add_argument("--precision", default="auto")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this renderer runs on one GPU; views shard across "
+                          "GPUs with no exchange, but only --gpus 1 is measured"}))
+        return 2
+
+    import torch
+    w = make_workload(args.workload)
+    ctx = X.Context(0)
+    scene = (X.Scene.from_voxels(w.voxels, w.deformation) if w.voxels is not None
+             else X.Scene.from_dict(w.obj, w.deformation))
+    lo, hi, empty = scene.clip_box()
+    out = torch.empty((len(w.angles), w.resolution, w.resolution), dtype=torch.float32,
+                      device="cuda:0")
+    kw = dict(integrator=w.integrator, precision=args.precision, R=w.R, fov=w.fov)
+
+    for _ in range(args.warmup):
+        st = X.render_device(ctx, scene, w.angles, w.resolution, w.ds, out.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = X.render_device(ctx, scene, w.angles, w.resolution, w.ds, out.data_ptr(), **kw)
+        kernel_ms.append(st["kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    projections = w.pixels / (512.0 * 512.0)
+    value = projections / (ms_per_step / 1000.0)
+
+    n_inbox = inbox_samples(w, lo, hi) if not empty else 0
+    k_ms = float(np.mean(kernel_ms))
+    samples_per_s = n_inbox / (ms_per_step / 1000.0)
+    prec = {1: "guarded", 2: "fp64"}.get(st["precision_used"], "auto")
+    if args.workload in BYTES_PER_SAMPLE:
+        achieved = BYTES_PER_SAMPLE[args.workload] * n_inbox / (k_ms / 1000.0) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        achieved = FLOPS_PER_SAMPLE[args.workload] * n_inbox / (k_ms / 1000.0) / 1e12
+        peak = FP32_PEAK_TFLOPS
+        roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
+    roof["kernel"] = "march_kernel<%s,%s>" % ("PREC_FP64" if prec == "fp64" else "PREC_GUARDED",
+                                             "INT_SIMPLE" if w.integrator == "simple"
+                                             else "INT_HIERARCHICAL")
+    roof["kernel_ms"] = k_ms
+    roof["per_sample"] = (f"{BYTES_PER_SAMPLE[args.workload]:.0f} B" if args.workload
+                          in BYTES_PER_SAMPLE else f"{FLOPS_PER_SAMPLE[args.workload]:.0f} flop")
+    roof["samples_per_launch"] = n_inbox
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "512² projections/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {len(w.angles)} views x {w.resolution}^2, "
+                               f"ds={w.ds:g}" + (f", grid {w.voxels.shape[0]}^3 f32"
+                                                 if w.voxels is not None else ""),
+                   "precision": prec, "integrator": w.integrator},
+        "ray_samples_per_s": samples_per_s,
+        "inbox_samples": n_inbox,
+        "reevaluated": st["reevaluated"],
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(line))
+    scene.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
