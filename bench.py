@@ -181,9 +181,14 @@ def main() -> int:
         peak = FP32_PEAK_TFLOPS
         roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None}
-    roof["kernel"] = "march_kernel<%s,%s>" % ("PREC_FP64" if prec == "fp64" else "PREC_GUARDED",
-                                             "INT_SIMPLE" if w.integrator == "simple"
-                                             else "INT_HIERARCHICAL")
+    if w.voxels is not None and prec != "fp64" and w.integrator == "simple":
+        layout = os.environ.get("XRP_VOXEL_LAYOUT", "brick4")
+        roof["kernel"] = "voxel_kernel<%s>" % ("VOX_LINEAR" if layout == "linear"
+                                               else "VOX_BRICK4")
+    else:
+        roof["kernel"] = "march_kernel<%s,%s>" % (
+            "PREC_FP64" if prec == "fp64" else "PREC_GUARDED",
+            "INT_SIMPLE" if w.integrator == "simple" else "INT_HIERARCHICAL")
     roof["kernel_ms"] = k_ms
     roof["per_sample"] = (f"{BYTES_PER_SAMPLE[args.workload]:.0f} B" if args.workload
                           in BYTES_PER_SAMPLE else f"{FLOPS_PER_SAMPLE[args.workload]:.0f} flop")

@@ -126,6 +126,35 @@ def test_voxel_reduced(ctx, precision):
     assert err <= TOL
 
 
+@pytest.fixture(scope="module")
+def blob256():
+    return W.blob_grid(256)
+
+
+@pytest.mark.parametrize("layout", ["brick4", "linear"])
+def test_voxel_full_config(ctx, blob256, layout, monkeypatch):
+    """The 256^3 blob grid at 512^2 through the dedicated voxel kernel,
+    8 of the benchmark's 64 views, against the FP64 oracle."""
+    monkeypatch.setenv("XRP_VOXEL_LAYOUT", layout)
+    w = W.voxel(grid=blob256)
+    angles = w.angles[:8]
+    err, st, img, ref = _compare(ctx, None, angles, w.resolution, w.ds,
+                                 deformation=w.deformation, voxels=blob256, audit=False)
+    assert err <= TOL
+    assert ref.min() < 0.6  # the blobs absorb
+
+
+def test_voxel_boundary_ties(ctx):
+    """Axis-aligned views put samples on the grid faces x = +-1 exactly;
+    the per-ray FP64 range must count them as the reference does."""
+    grid = np.ones((9, 9, 9), dtype=np.float32)
+    for ds in (0.025, 0.0125, 0.01):
+        err, st, img, ref = _compare(ctx, None, [(0.0, 90.0), (90.0, 90.0), (180.0, 90.0)],
+                                     33, ds, deformation=W.VOXEL_DEFORMATION, voxels=grid,
+                                     audit=False)
+        assert err <= TOL
+
+
 def test_polar_pole_views_are_blank(ctx):
     """polar 0 gives eye on +z, so LookAt's side vector normalizes a zero
     vector and the camera is NaN: every density test is false in the

@@ -31,6 +31,7 @@ DEVICE = ["-x", "hip", "--offload-arch=" + ARCH,
 
 SOURCES = [
     ("march.hip", True),
+    ("voxel.hip", True),
     ("api.cpp", False),
     ("scene.cpp", False),
     ("camera.cpp", False),

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -305,6 +306,51 @@ struct RenderJob {
   int integrator, precision, audit;
 };
 
+// [z][x][y] grid -> 4x4x4 bricks ordered (bz, bx, by), voxels (lz, lx, ly)
+// inside a brick; padding voxels are zero and never read.
+std::vector<float> brick4(const std::vector<float>& v, int nx, int ny, int nz) {
+  int bx = (nx + 3) / 4, by = (ny + 3) / 4, bz = (nz + 3) / 4;
+  std::vector<float> out((size_t)bx * by * bz * 64, 0.0f);
+  for (int z = 0; z < nz; ++z)
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y) {
+        size_t b = ((size_t)(z >> 2) * bx + (x >> 2)) * by + (y >> 2);
+        size_t l = (size_t)(z & 3) * 16 + (x & 3) * 4 + (y & 3);
+        out[b * 64 + l] = v[((size_t)z * nx + x) * ny + y];
+      }
+  return out;
+}
+
+// World -> object map of a chain of affine/linear/rigid deformations.
+bool compose_linear(const std::vector<xrp::HDeform>& defs, double M[9], double t[3]) {
+  double m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, v[3] = {0, 0, 0};
+  for (auto& f : defs) {
+    double Mi[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, ti[3] = {0, 0, 0};
+    if (f.kind == xrp::D_AFFINE) {
+      std::memcpy(Mi, f.p, sizeof Mi);
+    } else if (f.kind == xrp::D_LINEAR) {
+      const double* e = f.p;
+      double L[9] = {1 + e[0], e[5], e[4], e[5], 1 + e[1], e[3], e[4], e[3], 1 + e[2]};
+      std::memcpy(Mi, L, sizeof Mi);
+    } else if (f.kind == xrp::D_RIGID) {
+      for (int k = 0; k < 3; ++k) ti[k] = f.p[k];
+    } else {
+      return false;
+    }
+    double nm[9], nv[3];
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c)
+        nm[3 * r + c] = Mi[3 * r] * m[c] + Mi[3 * r + 1] * m[3 + c] + Mi[3 * r + 2] * m[6 + c];
+      nv[r] = Mi[3 * r] * v[0] + Mi[3 * r + 1] * v[1] + Mi[3 * r + 2] * v[2] + ti[r];
+    }
+    std::memcpy(m, nm, sizeof m);
+    std::memcpy(v, nv, sizeof v);
+  }
+  std::memcpy(M, m, sizeof m);
+  std::memcpy(t, v, sizeof v);
+  return true;
+}
+
 int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs) {
   std::lock_guard<std::mutex> g(g_registry_mu);
   auto it = sc->uploads.find(ctx);
@@ -334,10 +380,30 @@ int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs) {
     }
     out.push_back(p);
   }
+  if (sc->s->root.kind == xrp::K_VOXEL) {
+    // 4x4x4-brick copy for the voxel-root kernel (appended last).
+    const xrp::HNode& n = sc->s->root;
+    std::vector<float> br = brick4(*n.vox, n.nx, n.ny, n.nz);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, br.size() * sizeof(float));
+    if (e == hipSuccess)
+      e = hipMemcpy(p, br.data(), br.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (p) (void)hipFree(p);
+      for (void* q : out) (void)hipFree(q);
+      return fail(XRP_ERR_ALLOC, std::string("voxel upload failed: ") +
+                                     hipGetErrorString(e));
+    }
+    out.push_back(p);
+  }
   sc->uploads[ctx] = out;
   ctx->scenes.insert(sc);
   ptrs = out;
   return XRP_OK;
+}
+
+size_t mp_bricks(const xrp::HNode& n) {
+  return (size_t)((n.nx + 3) / 4) * ((n.ny + 3) / 4) * ((n.nz + 3) / 4);
 }
 
 int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
@@ -348,9 +414,17 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   const int res = job.res;
   HIPCHK(hipSetDevice(ctx->device));
 
+  // Voxel-root scenes under linear deformations take the dedicated voxel
+  // kernel (FP32 grid stepping, FP64 re-evaluation at the grid faces).
+  double vM[9], vt[3];
+  const bool vox_kernel =
+      s.root.kind == K_VOXEL && compose_linear(s.defs, vM, vt) &&
+      job.integrator == XRP_INTEGRATE_SIMPLE && !job.audit &&
+      job.precision != XRP_PRECISION_FP64 &&
+      (size_t)mp_bricks(s.root) * 64 < ((size_t)1 << 30);
   int precision = job.precision;
   if (precision == XRP_PRECISION_AUTO)
-    precision = s.has_voxels ? XRP_PRECISION_FP64 : XRP_PRECISION_GUARDED;
+    precision = (s.has_voxels && !vox_kernel) ? XRP_PRECISION_FP64 : XRP_PRECISION_GUARDED;
 
   // Sample table (cached per context).
   const double smin = job.R - kCubeHalfDiagonal, smax = job.R + kCubeHalfDiagonal;
@@ -396,7 +470,7 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     Bw = 2.0 * std::fabs(job.R) + kCubeHalfDiagonal + 1.0;
   }
   double delta = 0.0;
-  if (precision == XRP_PRECISION_GUARDED) {
+  if (precision == XRP_PRECISION_GUARDED && !vox_kernel) {
     double step = job.integrator == XRP_INTEGRATE_SIMPLE ? 32.0 * job.ds : 0.0;
     delta = compute_guards(s, Bw, step);
     // Tiny features against a coarse FP32 grid: fall back to FP64.
@@ -453,6 +527,31 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   if (clip == 2) mp.K = 0;  // zero density everywhere: T = flat_field
   mp.audit = job.audit;
   mp.out_view_stride = res * res;
+  int layout = VOX_BRICK4;
+  if (vox_kernel) {
+    const char* env = std::getenv("XRP_VOXEL_LAYOUT");
+    if (env && std::string(env) == "linear") layout = VOX_LINEAR;
+    mp.vgrid = (const float*)(layout == VOX_BRICK4 ? vptr.back() : vptr.front());
+    mp.vbytes = (uint32_t)(4ull * (layout == VOX_BRICK4
+                                       ? (size_t)mp_bricks(s.root) * 64
+                                       : (size_t)s.root.nx * s.root.ny * s.root.nz));
+    const HNode& vn = s.root;
+    mp.vn[0] = vn.nx;
+    mp.vn[1] = vn.ny;
+    mp.vn[2] = vn.nz;
+    mp.vbricks[0] = (vn.nx + 3) / 4;
+    mp.vbricks[1] = (vn.ny + 3) / 4;
+    mp.vbricks[2] = (vn.nz + 3) / 4;
+    std::memcpy(mp.M, vM, sizeof vM);
+    std::memcpy(mp.t, vt, sizeof vt);
+    double L = 0;
+    for (int r = 0; r < 3; ++r)
+      L = std::max(L, std::fabs(vM[3 * r]) + std::fabs(vM[3 * r + 1]) + std::fabs(vM[3 * r + 2]));
+    double N = std::max({vn.nx, vn.ny, vn.nz});
+    const double u = 1.0 / 16777216.0;
+    // FP32 anchor rounding (u*|G|) plus the FP32 step (4u * |B| * chunk*ds).
+    mp.vguard = (float)(4.0 * (u * (N + 4.0 * L * 0.5 * N * 32.0 * job.ds) + 1e-9));
+  }
 
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
   const int kMaxGridY = 65535;
@@ -463,7 +562,8 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     int prec = precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED;
     int integ = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
                                                              : INT_SIMPLE;
-    int e = launch_march(mp, prec, integ, ctx->stream);
+    int e = vox_kernel ? launch_voxel(mp, layout, ctx->stream)
+                       : launch_march(mp, prec, integ, ctx->stream);
     if (e != 0)
       return fail(XRP_ERR_DEVICE, std::string("kernel launch failed: ") +
                                       hipGetErrorString((hipError_t)e));

@@ -47,7 +47,19 @@ struct MarchParams {
   int32_t audit;           // evaluate every sample in FP64 as well
   int32_t view_offset;     // first view index of this launch
   int32_t out_view_stride; // floats between consecutive views in out
+  // Voxel-root kernel: world -> object affine map (the composed linear
+  // deformation chain), grid dims, brick layout and the grid-space guard.
+  const float* vgrid;      // linear [z][x][y] or 4^3-brick layout
+  int32_t vn[3];           // nx ny nz
+  int32_t vbricks[3];      // bricks per axis (x, y, z)
+  double M[9];             // row-major
+  double t[3];
+  float vguard;            // grid units
+  uint32_t vbytes;         // size of vgrid in bytes (buffer descriptor range)
 };
+
+enum VoxelLayout : int32_t { VOX_LINEAR = 0, VOX_BRICK4 = 1 };
+int launch_voxel(const MarchParams& p, int layout, void* stream);
 
 // Launch on the given stream; returns hipError_t as int.
 int launch_march(const MarchParams& p, int precision, int integrator,
