@@ -183,8 +183,8 @@ def main() -> int:
                 "frac": achieved / peak, "traffic": None}
     if w.voxels is not None and prec != "fp64" and w.integrator == "simple":
         layout = os.environ.get("XRP_VOXEL_LAYOUT", "brick4")
-        roof["kernel"] = "voxel_kernel<%s>" % ("VOX_LINEAR" if layout == "linear"
-                                               else "VOX_BRICK4")
+        roof["kernel"] = {"linear": "voxel_kernel<VOX_LINEAR>",
+                          "brick4": "voxel_kernel<VOX_BRICK4>"}.get(layout, "voxel_lds_kernel")
     else:
         roof["kernel"] = "march_kernel<%s,%s>" % (
             "PREC_FP64" if prec == "fp64" else "PREC_GUARDED",
@@ -207,6 +207,8 @@ def main() -> int:
         "ray_samples_per_s": samples_per_s,
         "inbox_samples": n_inbox,
         "reevaluated": st["reevaluated"],
+        "kernel_stats": {k: st[k] for k in ("staged_chunks", "staged_voxels",
+                                            "fallback_chunks")},
         "roofline": roof,
     }
     if not args.no_cpu:

@@ -100,6 +100,9 @@ typedef struct {
   double position_error;   /* FP32 object-space position error bound used */
   int32_t precision_used;  /* xrp_precision actually run */
   int32_t reserved;
+  uint64_t staged_chunks;   /* voxel kernel: sample chunks served from LDS */
+  uint64_t staged_voxels;   /* voxel kernel: voxels copied into LDS */
+  uint64_t fallback_chunks; /* voxel kernel: chunks gathered from global memory */
 } xrp_render_stats;
 
 int xrp_abi_version(void);

@@ -131,7 +131,7 @@ def blob256():
     return W.blob_grid(256)
 
 
-@pytest.mark.parametrize("layout", ["brick4", "linear"])
+@pytest.mark.parametrize("layout", ["lds", "brick4", "linear"])
 def test_voxel_full_config(ctx, blob256, layout, monkeypatch):
     """The 256^3 blob grid at 512^2 through the dedicated voxel kernel,
     8 of the benchmark's 64 views, against the FP64 oracle."""

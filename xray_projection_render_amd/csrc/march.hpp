@@ -22,6 +22,9 @@ enum StatSlot {
   ST_REEVAL = 1,       // guard-band samples re-evaluated in FP64
   ST_AUDIT_MISS = 2,   // audit: FP32-clear samples whose FP64 value differs
   ST_FINE = 3,         // hierarchical fine-step evaluations
+  ST_STAGED_CHUNKS = 4,   // LDS voxel kernel: chunks staged in LDS
+  ST_STAGED_VOXELS = 5,   //   voxels copied into LDS
+  ST_FALLBACK_CHUNKS = 6, //   chunks too large for LDS (gathered)
   ST_COUNT = 8
 };
 
@@ -56,9 +59,14 @@ struct MarchParams {
   double t[3];
   float vguard;            // grid units
   uint32_t vbytes;         // size of vgrid in bytes (buffer descriptor range)
+  const float* vlin;       // linear [z][x][y] copy (LDS staging source)
+  uint32_t vlin_bytes;
+  int32_t pad2_;
 };
 
-enum VoxelLayout : int32_t { VOX_LINEAR = 0, VOX_BRICK4 = 1 };
+// VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with
+// the 4^3-brick copy as the gather fallback for oversized sub-volumes.
+enum VoxelLayout : int32_t { VOX_LINEAR = 0, VOX_BRICK4 = 1, VOX_LDS = 2 };
 int launch_voxel(const MarchParams& p, int layout, void* stream);
 
 // Launch on the given stream; returns hipError_t as int.

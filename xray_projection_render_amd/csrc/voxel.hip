@@ -138,8 +138,9 @@ __global__ __launch_bounds__(256) void voxel_kernel(MarchParams p) {
   const int view = p.view_offset + blockIdx.y;
   const bool valid = i < p.res && j < p.res;
 
-  unsigned long long n_samples = 0;
-  double T = p.flat_field;
+  // Per lane: exact inside-sample range [ka, kb] and the grid-space line.
+  int ka = 0x7fffffff, kb = -1;
+  double A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
   if (valid) {
     const DevView& V = p.views[view];
     Ray r = make_ray(p, V, i, j);
@@ -160,53 +161,286 @@ __global__ __launch_bounds__(256) void voxel_kernel(MarchParams p) {
         hit = false;
       }
     }
-    long ka, kb;
+    long la, lb;
     // pad the analytic interval by one step on each side; exact_range
     // then trims it to the samples the reference counts as inside.
-    if (hit && ta <= tb && exact_range(p, r, ta - p.ds, tb + p.ds, ka, kb)) {
-      double A[3], B[3];
+    if (hit && ta <= tb && exact_range(p, r, ta - p.ds, tb + p.ds, la, lb)) {
+      ka = (int)la;
+      kb = (int)lb;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         double h = 0.5 * (double)(p.vn[c] - 1);
         A[c] = (Ao[c] + 1.0) * h;
         B[c] = Bo[c] * h;
       }
-      const float b0 = (float)B[0], b1 = (float)B[1], b2 = (float)B[2];
-      const float ds32 = (float)p.ds;
-      const double ds = p.ds, dm = p.dm;
-      const __amdgpu_buffer_rsrc_t rs = grid_rsrc(p.vgrid, p.vbytes);
-      for (long k0 = ka; k0 <= kb; k0 += kChunk) {
-        double sa = p.s_table[k0];
-        float a0 = (float)(A[0] + B[0] * sa), a1 = (float)(A[1] + B[1] * sa),
-              a2 = (float)(A[2] + B[2] * sa);
-        int n = (int)min((long)kChunk, kb - k0 + 1);
-        float acc = 0.0f, comp = 0.0f;  // Kahan-compensated chunk sum
-#pragma unroll 4
-        for (int jj = 0; jj < n; ++jj) {
+    }
+  }
+  // The wave walks one sample index at a time, so its 64 lanes gather from
+  // one thin slab of the grid (8x8 neighbouring rays at the same depth).
+  int wk0 = ka, wk1 = kb;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    wk0 = min(wk0, __shfl_xor(wk0, off, 64));
+    wk1 = max(wk1, __shfl_xor(wk1, off, 64));
+  }
+  double T = p.flat_field;
+  const float b0 = (float)B[0], b1 = (float)B[1], b2 = (float)B[2];
+  const float ds32 = (float)p.ds;
+  const double ds = p.ds, dm = p.dm;
+  const __amdgpu_buffer_rsrc_t rs = grid_rsrc(p.vgrid, p.vbytes);
+  for (int k0 = wk0; k0 <= wk1; k0 += kChunk) {
+    double sa = p.s_table[k0];
+    float a0 = (float)(A[0] + B[0] * sa), a1 = (float)(A[1] + B[1] * sa),
+          a2 = (float)(A[2] + B[2] * sa);
+    const int n = min(kChunk, wk1 - k0 + 1);
+    const int jlo = ka - k0, jhi = kb - k0;
+    float acc = 0.0f, comp = 0.0f;  // Kahan-compensated chunk sum
+    for (int jj = 0; jj < n; ++jj) {
+      if (jj >= jlo && jj <= jhi) {
+        float sj = (float)jj * ds32;
+        float gx = fmaf(b0, sj, a0), gy = fmaf(b1, sj, a1),
+              gz = fmaf(b2, sj, a2);
+        float rho = trilinear<LAYOUT>(p, rs, fmaxf(gx, 0.0f), fmaxf(gy, 0.0f),
+                                      fmaxf(gz, 0.0f));
+        float y = rho - comp;
+        float tt = acc + y;
+        comp = (tt - acc) - y;
+        acc = tt;
+      }
+    }
+    T += ((double)acc * dm) * ds;
+  }
+  if (valid)
+    p.out[(size_t)blockIdx.y * p.out_view_stride + (size_t)i * p.res + j] =
+        (float)exp(-T);
+  if (p.stats) {
+    unsigned long long a = wave_sum(kb >= ka ? (unsigned long long)(kb - ka + 1) : 0ull);
+    if (lane == 0 && a) atomicAdd(p.stats + ST_SAMPLES, a);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged variant.  One 64-lane workgroup = one 8x8 ray tile.  For each
+// chunk of kLdsChunk sample indices the wave takes the box of voxels its
+// rays can touch (union of every active lane's segment, +1 for the upper
+// trilinear corner), copies it from the linear [z][x][y] grid into LDS with
+// row-contiguous loads, and interpolates from LDS.  A box larger than the
+// LDS slot falls back to brick-layout gathers for that chunk.
+// ---------------------------------------------------------------------------
+constexpr int kLdsFloats = 4096;  // 16 KB per wave
+constexpr int kLdsChunk = 16;
+
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(64) void voxel_lds_kernel(MarchParams p) {
+  __shared__ float tile[kLdsFloats];
+  const int lane = threadIdx.x;
+  // Workgroup b runs on XCD b % 8.  Give each XCD a contiguous run of
+  // tiles so its L2 holds one band of the image's footprint.
+  const int tiles_j = (p.res + 7) >> 3;
+  const int ntiles = tiles_j * tiles_j;
+  int logical = blockIdx.x;
+  if ((ntiles & 7) == 0) logical = (blockIdx.x & 7) * (ntiles >> 3) + (blockIdx.x >> 3);
+  const int ti = logical / tiles_j, tj = logical - ti * tiles_j;
+  const int i = ti * 8 + (lane >> 3);
+  const int j = tj * 8 + (lane & 7);
+  const int view = p.view_offset + blockIdx.y;
+  const bool valid = i < p.res && j < p.res;
+
+  int ka = 0x7fffffff, kb = -1;
+  double A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+  if (valid) {
+    const DevView& V = p.views[view];
+    Ray r = make_ray(p, V, i, j);
+    double Ao[3], Bo[3], ta = -INFINITY, tb = INFINITY;
+    bool hit = r.d[0] == r.d[0] && r.d[1] == r.d[1] && r.d[2] == r.d[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Ao[c] = p.M[3 * c] * r.o[0] + p.M[3 * c + 1] * r.o[1] +
+              p.M[3 * c + 2] * r.o[2] + p.t[c];
+      Bo[c] = p.M[3 * c] * r.d[0] + p.M[3 * c + 1] * r.d[1] +
+              p.M[3 * c + 2] * r.d[2];
+      if (Bo[c] != 0.0) {
+        double t1 = (-1.0 - Ao[c]) / Bo[c], t2 = (1.0 - Ao[c]) / Bo[c];
+        ta = fmax(ta, fmin(t1, t2));
+        tb = fmin(tb, fmax(t1, t2));
+      } else if (!(Ao[c] >= -1.0 && Ao[c] <= 1.0)) {
+        hit = false;
+      }
+    }
+    long la, lb;
+    if (hit && ta <= tb && exact_range(p, r, ta - p.ds, tb + p.ds, la, lb)) {
+      ka = (int)la;
+      kb = (int)lb;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double h = 0.5 * (double)(p.vn[c] - 1);
+        A[c] = (Ao[c] + 1.0) * h;
+        B[c] = Bo[c] * h;
+      }
+    }
+  }
+  int wk0 = ka, wk1 = kb;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    wk0 = min(wk0, __shfl_xor(wk0, off, 64));
+    wk1 = max(wk1, __shfl_xor(wk1, off, 64));
+  }
+  const int nx = p.vn[0], ny = p.vn[1], nz = p.vn[2];
+  double T = p.flat_field;
+  const float b0 = (float)B[0], b1 = (float)B[1], b2 = (float)B[2];
+  const float ds32 = (float)p.ds;
+  const double ds = p.ds, dm = p.dm;
+  const __amdgpu_buffer_rsrc_t rs_brick = grid_rsrc(p.vgrid, p.vbytes);
+  const __amdgpu_buffer_rsrc_t rs_lin = grid_rsrc(p.vlin, p.vlin_bytes);
+  unsigned long long n_staged = 0, n_chunks = 0, n_fallback = 0;
+  for (int k0 = wk0; k0 <= wk1; k0 += kLdsChunk) {
+    const double sa = p.s_table[k0];
+    const float a0 = (float)(A[0] + B[0] * sa), a1 = (float)(A[1] + B[1] * sa),
+                a2 = (float)(A[2] + B[2] * sa);
+    const int n = min(kLdsChunk, wk1 - k0 + 1);
+    const int jlo = ka - k0, jhi = kb - k0;
+    const int j0 = max(jlo, 0), j1 = min(jhi, n - 1);
+    const bool act = j0 <= j1;
+    // This lane's footprint: the segment between its first and last
+    // sample of the chunk (positions are monotone in the sample index).
+    float lo0 = 3e38f, lo1 = 3e38f, lo2 = 3e38f, hi0 = -3e38f, hi1 = -3e38f,
+          hi2 = -3e38f;
+    if (act) {
+      const float s0 = (float)j0 * ds32, s1 = (float)j1 * ds32;
+      float x0 = fmaf(b0, s0, a0), x1 = fmaf(b0, s1, a0);
+      float y0 = fmaf(b1, s0, a1), y1 = fmaf(b1, s1, a1);
+      float z0 = fmaf(b2, s0, a2), z1 = fmaf(b2, s1, a2);
+      lo0 = fminf(x0, x1);
+      hi0 = fmaxf(x0, x1);
+      lo1 = fminf(y0, y1);
+      hi1 = fmaxf(y0, y1);
+      lo2 = fminf(z0, z1);
+      hi2 = fmaxf(z0, z1);
+    }
+    lo0 = wave_minf(lo0);
+    lo1 = wave_minf(lo1);
+    lo2 = wave_minf(lo2);
+    hi0 = wave_maxf(hi0);
+    hi1 = wave_maxf(hi1);
+    hi2 = wave_maxf(hi2);
+    if (!(lo0 <= hi0)) continue;  // no lane samples in this chunk
+    const int X0 = max(0, (int)floorf(fmaxf(lo0, 0.0f))),
+              X1 = min(nx - 1, (int)floorf(fminf(hi0, (float)nx)) + 1);
+    const int Y0 = max(0, (int)floorf(fmaxf(lo1, 0.0f))),
+              Y1 = min(ny - 1, (int)floorf(fminf(hi1, (float)ny)) + 1);
+    const int Z0 = max(0, (int)floorf(fmaxf(lo2, 0.0f))),
+              Z1 = min(nz - 1, (int)floorf(fminf(hi2, (float)nz)) + 1);
+    // Samples lie inside the grid (exact_range), so X0 <= X1; the min()
+    // only keeps every loop below finite whatever the inputs.
+    const int bx = X1 - min(X0, X1) + 1, by = Y1 - min(Y0, Y1) + 1,
+              bz = Z1 - min(Z0, Z1) + 1;
+    float acc = 0.0f, comp = 0.0f;  // Kahan-compensated chunk sum
+    if (by <= 64 && bx * by * bz <= kLdsFloats) {
+      n_staged += (unsigned long long)(bx * by * bz);
+      ++n_chunks;
+      // Stage: lane = (row offset, y); rpi rows per pass.
+      const int rpi = 64 / by;
+      const int ro = lane / by, ly = lane - ro * by;
+      const int R = bz * bx;
+      if (ro < rpi) {
+        int z = 0, x = ro;
+        while (x >= bx) {
+          x -= bx;
+          ++z;
+        }
+        for (int rr = ro; rr < R; rr += rpi) {
+          unsigned g = ((unsigned)(Z0 + z) * (unsigned)nx + (unsigned)(X0 + x)) *
+                           (unsigned)ny + (unsigned)(Y0 + ly);
+          tile[rr * by + ly] = __uint_as_float(
+              __builtin_amdgcn_raw_buffer_load_b32(rs_lin, (int)(g * 4u), 0, 0));
+          x += rpi;
+          while (x >= bx) {
+            x -= bx;
+            ++z;
+          }
+        }
+      }
+      __syncthreads();
+      const int sxy = bx * by;
+      for (int jj = 0; jj < n; ++jj) {
+        if (jj >= jlo && jj <= jhi) {
           float sj = (float)jj * ds32;
-          float gx = fmaf(b0, sj, a0), gy = fmaf(b1, sj, a1),
-                gz = fmaf(b2, sj, a2);
-          float rho = trilinear<LAYOUT>(p, rs, fmaxf(gx, 0.0f), fmaxf(gy, 0.0f),
-                                        fmaxf(gz, 0.0f));
+          float gx = fmaxf(fmaf(b0, sj, a0), 0.0f), gy = fmaxf(fmaf(b1, sj, a1), 0.0f),
+                gz = fmaxf(fmaf(b2, sj, a2), 0.0f);
+          int x0 = min((int)gx, nx - 1), y0 = min((int)gy, ny - 1),
+              z0 = min((int)gz, nz - 1);
+          float wx = gx - (float)x0, wy = gy - (float)y0, wz = gz - (float)z0;
+          int dx = min(x0 + 1, nx - 1) - x0, dy = min(y0 + 1, ny - 1) - y0,
+              dz = min(z0 + 1, nz - 1) - z0;
+          int base = ((z0 - Z0) * bx + (x0 - X0)) * by + (y0 - Y0);
+          int ox = dx * by, oz = dz * sxy;
+          float v000 = tile[base], v010 = tile[base + dy];
+          float v100 = tile[base + ox], v110 = tile[base + ox + dy];
+          float v001 = tile[base + oz], v011 = tile[base + oz + dy];
+          float v101 = tile[base + oz + ox], v111 = tile[base + oz + ox + dy];
+          float v00 = fmaf(wz, v001 - v000, v000);
+          float v01 = fmaf(wz, v011 - v010, v010);
+          float v10 = fmaf(wz, v101 - v100, v100);
+          float v11 = fmaf(wz, v111 - v110, v110);
+          float v0 = fmaf(wy, v01 - v00, v00);
+          float v1 = fmaf(wy, v11 - v10, v10);
+          float rho = fmaf(wx, v1 - v0, v0);
           float y = rho - comp;
           float tt = acc + y;
           comp = (tt - acc) - y;
           acc = tt;
         }
-        T += ((double)acc * dm) * ds;
-        n_samples += (unsigned long long)n;
+      }
+      __syncthreads();
+    } else {
+      ++n_fallback;
+      for (int jj = 0; jj < n; ++jj) {
+        if (jj >= jlo && jj <= jhi) {
+          float sj = (float)jj * ds32;
+          float gx = fmaf(b0, sj, a0), gy = fmaf(b1, sj, a1), gz = fmaf(b2, sj, a2);
+          float rho = trilinear<VOX_BRICK4>(p, rs_brick, fmaxf(gx, 0.0f),
+                                            fmaxf(gy, 0.0f), fmaxf(gz, 0.0f));
+          float y = rho - comp;
+          float tt = acc + y;
+          comp = (tt - acc) - y;
+          acc = tt;
+        }
       }
     }
+    T += ((double)acc * dm) * ds;
+  }
+  if (valid)
     p.out[(size_t)blockIdx.y * p.out_view_stride + (size_t)i * p.res + j] =
         (float)exp(-T);
-  }
   if (p.stats) {
-    unsigned long long a = wave_sum(n_samples);
-    if (lane == 0 && a) atomicAdd(p.stats + ST_SAMPLES, a);
+    unsigned long long a = wave_sum(kb >= ka ? (unsigned long long)(kb - ka + 1) : 0ull);
+    if (lane == 0) {
+      if (a) atomicAdd(p.stats + ST_SAMPLES, a);
+      // wave-uniform staging counters
+      if (n_chunks) atomicAdd(p.stats + ST_STAGED_CHUNKS, n_chunks);
+      if (n_staged) atomicAdd(p.stats + ST_STAGED_VOXELS, n_staged);
+      if (n_fallback) atomicAdd(p.stats + ST_FALLBACK_CHUNKS, n_fallback);
+    }
   }
 }
 
 int launch_voxel(const MarchParams& p, int layout, void* stream) {
+  if (layout == VOX_LDS) {
+    int tiles = (p.res + 7) / 8;
+    hipLaunchKernelGGL(voxel_lds_kernel, dim3(tiles * tiles, p.n_views), dim3(64), 0,
+                       (hipStream_t)stream, p);
+    return (int)hipGetLastError();
+  }
   dim3 block(256);
   int tiles = (p.res + 15) / 16;
   dim3 grid(tiles * tiles, p.n_views);
