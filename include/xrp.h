@@ -82,8 +82,18 @@ typedef struct {
   int32_t precision;         /* xrp_precision */
   int32_t audit;             /* 1: also evaluate every sample in FP64 and count
                                 FP32 decisions that disagree (slow; testing) */
+  int32_t output_format;     /* xrp_output_format */
+  int32_t transparency;      /* XRP_OUTPUT_GRAY8_ALPHA8: alpha 0 where T == 1 */
   int32_t reserved;
 } xrp_render_params;
+
+/* Output pixel formats, layout [view][i][j] (i = image x, main.go:375-378).
+ *  F32            - float32 transmittance exp(-T).
+ *  GRAY8_ALPHA8   - two bytes per pixel as the reference's PNG stores them
+ *                   (main.go:481-498): gray = uint16(val*0xffff) >> 8 from
+ *                   the FP64 transmittance, alpha = 0 under --transparency
+ *                   where val >= 1, else 255. */
+typedef enum { XRP_OUTPUT_F32 = 0, XRP_OUTPUT_GRAY8_ALPHA8 = 1 } xrp_output_format;
 
 /* Fills p with the reference's CLI defaults (main.go:651-761):
  * resolution 512, ds -1 (caller must resolve; see xrp_scene_auto_ds),
@@ -120,6 +130,12 @@ int xrp_context_synchronize(xrp_context *ctx);
  * resolves relative voxel_grid raw paths (NULL = current directory). */
 int xrp_scene_create(const char *object_text, const char *deformation_text,
                      int number_mode, const char *base_dir, xrp_scene **out);
+/* Scene from files as load_object / load_deformation read them
+ * (main.go:50-120): the last four characters pick YAML ("yaml", integer
+ * literals stay int) or JSON ("json", every number float64); raw voxel
+ * paths resolve against the working directory. */
+int xrp_scene_load_files(const char *object_path, const char *deformation_path,
+                         xrp_scene **out);
 /* Voxel-grid scene from caller memory: rho[z*nx*ny + x*ny + y] in [0,1]. */
 int xrp_scene_create_voxels(const float *rho, int nx, int ny, int nz,
                             const char *deformation_text, int number_mode,
@@ -146,14 +162,14 @@ int xrp_camera_from_angles(double azimuthal_deg, double polar_deg, double R,
 int xrp_sample_count(double smin, double smax, double ds, int64_t *count);
 
 /* ---- rendering ------------------------------------------------------------ */
-/* out: n_views * res * res float32, [view][i][j], i = image x (host memory). */
+/* out: n_views * res * res pixels in p->output_format (host memory). */
 int xrp_render(xrp_context *ctx, xrp_scene *scene, const xrp_render_params *p,
-               const xrp_view_angle *views, int n_views, float *out,
+               const xrp_view_angle *views, int n_views, void *out,
                xrp_render_stats *stats);
 /* Same, out is device memory on the context's device; returns when done. */
 int xrp_render_device(xrp_context *ctx, xrp_scene *scene,
                       const xrp_render_params *p, const xrp_view_angle *views,
-                      int n_views, float *out_device, xrp_render_stats *stats);
+                      int n_views, void *out_device, xrp_render_stats *stats);
 
 /* Explicit-camera voxel render replacing RenderVolumeProjectionsCUDA
  * (cuda_backend.h:70-112) with the CPU renderer's semantics: software
@@ -169,6 +185,13 @@ int xrp_render_volume_projections(xrp_context *ctx, const float *volume, int nx,
                                   int ny, int nz, const xrp_camera *cameras,
                                   int n_cameras, int image_res, double ds,
                                   double flat_field, float *out_images);
+
+/* Volume export of main.go:549-635 (--export_volume): density (deformation
+ * and density_multiplier applied) at x = i/res*2-1, the i/N convention of
+ * main.go:208-214; divided by the maximum, times 255, converted to a byte
+ * as Go does.  out[k*res*res + i*res + j], res^3 bytes, host memory. */
+int xrp_export_volume(xrp_context *ctx, xrp_scene *scene, int res,
+                      double density_multiplier, uint8_t *out);
 
 /* ---- reference JSON API (api.go:82-180) ----------------------------------- */
 /* Renders as the reference's RenderProjections does (PNG frames,

@@ -3,10 +3,13 @@
 Drop-in for the per-pixel line-integral path of
 igrega348/xray_projection_render: camera/ray generation, the fixed-step
 Beer-Lambert march over the primitive tree and deformation fields, and the
-trilinear voxel sampler run as gfx950 kernels in libxrp.so.
+trilinear voxel sampler run as gfx950 kernels in libxrp.so.  XRayRenderer
+keeps the reference's Python interface.
 """
-from ._lib import (Context, Scene, XrpError, camera_from_angles, lib, render,
-                   render_device, render_volume_projections, sample_count)
+from ._lib import (Context, Scene, XrpError, camera_from_angles, export_volume, lib,
+                   render, render_device, render_volume_projections, sample_count)
+from .xray_renderer import XRayRenderer
 
-__all__ = ["Context", "Scene", "XrpError", "camera_from_angles", "lib", "render",
-           "render_device", "render_volume_projections", "sample_count"]
+__all__ = ["Context", "Scene", "XRayRenderer", "XrpError", "camera_from_angles",
+           "export_volume", "lib", "render", "render_device", "render_volume_projections",
+           "sample_count"]

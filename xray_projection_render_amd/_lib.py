@@ -50,7 +50,8 @@ class RenderParams(ctypes.Structure):
         ("fov_deg", ctypes.c_double), ("flat_field", ctypes.c_double),
         ("density_multiplier", ctypes.c_double),
         ("integrator", ctypes.c_int32), ("precision", ctypes.c_int32),
-        ("audit", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("audit", ctypes.c_int32), ("output_format", ctypes.c_int32),
+        ("transparency", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -110,12 +111,16 @@ def lib() -> ctypes.CDLL:
         "xrp_camera_from_angles": (I, [D, D, D, DP, DP]),
         "xrp_sample_count": (I, [D, D, D, ctypes.POINTER(ctypes.c_int64)]),
         "xrp_render": (I, [P, P, ctypes.POINTER(RenderParams), ctypes.POINTER(ViewAngle),
-                           I, FP, ctypes.POINTER(RenderStats)]),
+                           I, P, ctypes.POINTER(RenderStats)]),
         "xrp_render_device": (I, [P, P, ctypes.POINTER(RenderParams),
                                   ctypes.POINTER(ViewAngle), I, P,
                                   ctypes.POINTER(RenderStats)]),
         "xrp_render_volume_projections": (I, [P, FP, I, I, I, ctypes.POINTER(Camera), I, I,
                                               D, D, FP]),
+        "xrp_export_volume": (I, [P, P, I, D, ctypes.POINTER(ctypes.c_uint8)]),
+        "xrp_scene_load_files": (I, [ctypes.c_char_p, ctypes.c_char_p, PP]),
+        "RenderProjections": (ctypes.POINTER(ctypes.c_char), [ctypes.c_char_p]),
+        "FreeString": (None, [ctypes.POINTER(ctypes.c_char)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -198,6 +203,15 @@ class Scene:
                              numbers, base_dir)
 
     @classmethod
+    def from_files(cls, object_path: str, deformation_path: Optional[str] = None) -> "Scene":
+        """load_object / load_deformation (main.go:50-120): YAML or JSON by extension."""
+        p = ctypes.c_void_p()
+        check(lib().xrp_scene_load_files(object_path.encode(),
+                                          deformation_path.encode() if deformation_path else None,
+                                          ctypes.byref(p)))
+        return cls(p)
+
+    @classmethod
     def from_voxels(cls, rho: np.ndarray, deformation: Optional[Dict] = None,
                     numbers: str = "yaml") -> "Scene":
         """rho shaped (nz, nx, ny): flat index z*nx*ny + x*ny + y."""
@@ -247,7 +261,8 @@ class Scene:
 def make_params(resolution: int, ds: float, R: float = 4.0, fov: float = 40.0,
                 flat_field: float = 0.0, density_multiplier: float = 1.0,
                 integrator: str = "simple", precision: str = "auto",
-                audit: bool = False) -> RenderParams:
+                audit: bool = False, output: str = "f32",
+                transparency: bool = False) -> RenderParams:
     p = RenderParams()
     lib().xrp_default_params(ctypes.byref(p))
     p.resolution = resolution
@@ -259,6 +274,8 @@ def make_params(resolution: int, ds: float, R: float = 4.0, fov: float = 40.0,
     p.integrator = INTEGRATORS[integrator]
     p.precision = PRECISIONS[precision]
     p.audit = 1 if audit else 0
+    p.output_format = {"f32": 0, "gray8_alpha8": 1}[output]
+    p.transparency = 1 if transparency else 0
     return p
 
 
@@ -272,12 +289,16 @@ def _views(angles: Sequence[Tuple[float, float]]):
 
 def render(ctx: Context, scene: Scene, angles: Sequence[Tuple[float, float]],
            resolution: int, ds: float, **kw) -> Tuple[np.ndarray, Dict]:
-    """Transmittance images [view][i][j] (float32) into host memory."""
+    """Transmittance images [view][i][j] (float32) into host memory, or
+    (gray, alpha) byte pairs [view][i][j][2] with output="gray8_alpha8"."""
     p = make_params(resolution, ds, **kw)
-    out = np.empty((len(angles), resolution, resolution), dtype=np.float32)
+    if p.output_format == 1:
+        out = np.empty((len(angles), resolution, resolution, 2), dtype=np.uint8)
+    else:
+        out = np.empty((len(angles), resolution, resolution), dtype=np.float32)
     st = RenderStats()
     check(lib().xrp_render(ctx.ptr, scene.ptr, ctypes.byref(p), _views(angles),
-                           len(angles), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                           len(angles), out.ctypes.data_as(ctypes.c_void_p),
                            ctypes.byref(st)))
     return out, st.as_dict()
 
@@ -290,6 +311,25 @@ def render_device(ctx: Context, scene: Scene, angles: Sequence[Tuple[float, floa
     check(lib().xrp_render_device(ctx.ptr, scene.ptr, ctypes.byref(p), _views(angles),
                                   len(angles), ctypes.c_void_p(out_ptr), ctypes.byref(st)))
     return st.as_dict()
+
+
+def export_volume(ctx: Context, scene: Scene, res: int,
+                  density_multiplier: float = 1.0) -> np.ndarray:
+    """volume.raw bytes of main.go:549-635, shaped (res, res, res) = (z, x, y)."""
+    out = np.empty((res, res, res), dtype=np.uint8)
+    check(lib().xrp_export_volume(ctx.ptr, scene.ptr, res, density_multiplier,
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    return out
+
+
+def render_projections_json(params: Dict) -> Dict:
+    """The library's RenderProjections (api.go:82-180) on a parameter dict."""
+    L = lib()
+    ptr = L.RenderProjections(json.dumps(params).encode())
+    try:
+        return json.loads(ctypes.string_at(ptr).decode())
+    finally:
+        L.FreeString(ptr)
 
 
 def render_volume_projections(ctx: Context, volume: np.ndarray,

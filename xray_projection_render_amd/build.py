@@ -54,7 +54,7 @@ def build(verbose: bool = True) -> str:
         objs = list(ex.map(lambda sd: _compile(*sd), srcs))
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, "-shared", "-o", tmp] + objs +
-                   ["--offload-arch=" + ARCH, "-lpthread"], check=True)
+                   ["--offload-arch=" + ARCH, "-lpthread", "-lz"], check=True)
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

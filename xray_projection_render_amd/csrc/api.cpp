@@ -69,6 +69,11 @@ struct xrp_scene {
   std::map<xrp_context*, std::vector<void*>> uploads;  // voxel grids per context
 };
 
+namespace xrp {
+const HNode& scene_root(const xrp_scene* s) { return s->s->root; }
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace xrp
+
 #define HIPCHK(expr)                                                        \
   do {                                                                      \
     hipError_t e_ = (expr);                                                 \
@@ -304,7 +309,10 @@ struct RenderJob {
   int res;
   double ds, R, fov, flat_field, dm;
   int integrator, precision, audit;
+  int output = 0, transparency = 0;
 };
+
+size_t pixel_bytes(int output) { return output == XRP_OUTPUT_GRAY8_ALPHA8 ? 2 : 4; }
 
 // [z][x][y] grid -> 4x4x4 bricks ordered (bz, bx, by), voxels (lz, lx, ly)
 // inside a brick; padding voxels are zero and never read.
@@ -407,7 +415,7 @@ size_t mp_bricks(const xrp::HNode& n) {
 }
 
 int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
-                float* out_dev, xrp_render_stats* st) {
+                void* out_dev, xrp_render_stats* st) {
   using namespace xrp;
   Scene& s = *sc->s;
   const int nv = (int)job.views.size();
@@ -527,6 +535,8 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   if (clip == 2) mp.K = 0;  // zero density everywhere: T = flat_field
   mp.audit = job.audit;
   mp.out_view_stride = res * res;
+  mp.out_mode = job.output;
+  mp.transparency = job.transparency;
   // 4^3-brick gathers measured fastest on the 256^3 voxel workload
   // (3.0k projections/s vs 2.2k LDS-staged, 1.6k linear); the others stay
   // selectable through XRP_VOXEL_LAYOUT=lds|linear for comparison.
@@ -564,7 +574,10 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   for (int v0 = 0; v0 < nv; v0 += kMaxGridY) {
     mp.view_offset = v0;
     mp.n_views = std::min(kMaxGridY, nv - v0);
-    mp.out = out_dev + (size_t)v0 * res * res;
+    if (job.output == XRP_OUTPUT_GRAY8_ALPHA8)
+      mp.out8 = (unsigned char*)out_dev + (size_t)v0 * res * res * 2;
+    else
+      mp.out = (float*)out_dev + (size_t)v0 * res * res;
     int prec = precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED;
     int integ = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
                                                              : INT_SIMPLE;
@@ -621,6 +634,8 @@ int check_params(const xrp_render_params* p, int n_views) {
     return fail(XRP_ERR_ARG, "unknown integrator");
   if (p->precision < XRP_PRECISION_AUTO || p->precision > XRP_PRECISION_FP64)
     return fail(XRP_ERR_ARG, "unknown precision");
+  if (p->output_format != XRP_OUTPUT_F32 && p->output_format != XRP_OUTPUT_GRAY8_ALPHA8)
+    return fail(XRP_ERR_ARG, "unknown output_format");
   double K = 2 * xrp::kCubeHalfDiagonal / p->ds;
   if (K > 5e7) return fail(XRP_ERR_ARG, "ds too small: more than 5e7 samples per ray");
   return XRP_OK;
@@ -638,6 +653,8 @@ RenderJob make_job(const xrp_render_params* p, const xrp_view_angle* views,
   job.integrator = p->integrator;
   job.precision = p->precision;
   job.audit = p->audit;
+  job.output = p->output_format;
+  job.transparency = p->transparency;
   job.views.resize(n_views);
   for (int v = 0; v < n_views; ++v) {
     xrp::DevView& dv = job.views[v];
@@ -654,7 +671,7 @@ extern "C" {
 
 int xrp_render_device(xrp_context* ctx, xrp_scene* sc,
                       const xrp_render_params* p, const xrp_view_angle* views,
-                      int n_views, float* out_device, xrp_render_stats* st) {
+                      int n_views, void* out_device, xrp_render_stats* st) {
   if (!ctx || !sc || !views || !out_device)
     return fail(XRP_ERR_NULL, "NULL argument");
   int rc = check_params(p, n_views);
@@ -665,19 +682,73 @@ int xrp_render_device(xrp_context* ctx, xrp_scene* sc,
 }
 
 int xrp_render(xrp_context* ctx, xrp_scene* sc, const xrp_render_params* p,
-               const xrp_view_angle* views, int n_views, float* out,
+               const xrp_view_angle* views, int n_views, void* out,
                xrp_render_stats* st) {
   if (!ctx || !sc || !views || !out) return fail(XRP_ERR_NULL, "NULL argument");
   int rc = check_params(p, n_views);
   if (rc) return rc;
   std::lock_guard<std::mutex> g(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
-  size_t bytes = (size_t)n_views * p->resolution * p->resolution * sizeof(float);
+  size_t bytes = (size_t)n_views * p->resolution * p->resolution *
+                 pixel_bytes(p->output_format);
   HIPCHK(ctx->out.ensure(bytes));
   RenderJob job = make_job(p, views, n_views);
-  rc = render_core(ctx, sc, job, (float*)ctx->out.p, st);
+  rc = render_core(ctx, sc, job, ctx->out.p, st);
   if (rc) return rc;
   HIPCHK(hipMemcpy(out, ctx->out.p, bytes, hipMemcpyDeviceToHost));
+  return XRP_OK;
+}
+
+int xrp_export_volume(xrp_context* ctx, xrp_scene* sc, int res,
+                      double density_multiplier, uint8_t* out) {
+  using namespace xrp;
+  if (!ctx || !sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
+  if (res <= 0 || res > 2048) return fail(XRP_ERR_DIMS, "res must be in 1..2048");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  Scene& s = *sc->s;
+  std::vector<void*> vptr;
+  if (s.has_voxels) {
+    int rc = upload_voxels(ctx, sc, vptr);
+    if (rc) return rc;
+  }
+  size_t vi = 0;
+  for (auto& n : s.nodes)
+    if (n.kind == K_VOXEL) n.vox = (const float*)vptr[vi++];
+  const size_t n = (size_t)res * res * res;
+  HIPCHK(ctx->nodes.ensure(s.nodes.size() * sizeof(DevNode)));
+  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, s.ddefs.size()) * sizeof(DevDeform)));
+  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
+  DevBuf dens, bytes;
+  HIPCHK(dens.ensure(n * sizeof(double)));
+  hipError_t he = bytes.ensure(n);
+  if (he != hipSuccess) {
+    dens.release();
+    return fail(XRP_ERR_ALLOC, hipGetErrorString(he));
+  }
+  MarchParams mp;
+  std::memset(&mp, 0, sizeof mp);
+  mp.nodes = (const DevNode*)ctx->nodes.p;
+  mp.defs = (const DevDeform*)ctx->defs.p;
+  mp.n_defs = (int)s.ddefs.size();
+  mp.dm = density_multiplier;
+  he = hipMemcpyAsync(ctx->nodes.p, s.nodes.data(), s.nodes.size() * sizeof(DevNode),
+                      hipMemcpyHostToDevice, ctx->stream);
+  if (he == hipSuccess && !s.ddefs.empty())
+    he = hipMemcpyAsync(ctx->defs.p, s.ddefs.data(), s.ddefs.size() * sizeof(DevDeform),
+                        hipMemcpyHostToDevice, ctx->stream);
+  if (he == hipSuccess)
+    he = hipMemsetAsync(ctx->stats.p, 0, sizeof(unsigned long long), ctx->stream);
+  if (he == hipSuccess)
+    he = (hipError_t)launch_volume(mp, res, (double*)dens.p, (unsigned long long*)ctx->stats.p,
+                                   (unsigned char*)bytes.p, ctx->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(out, bytes.p, n, hipMemcpyDeviceToHost, ctx->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+  dens.release();
+  bytes.release();
+  if (he != hipSuccess)
+    return fail(XRP_ERR_DEVICE, std::string("volume export failed: ") + hipGetErrorString(he));
   return XRP_OK;
 }
 
@@ -722,7 +793,7 @@ int xrp_render_volume_projections(xrp_context* ctx, const float* volume, int nx,
       std::memcpy(dv.cam, cams[c].cam2world, sizeof dv.cam);
       job.views.push_back(dv);
     }
-    rc = render_core(ctx, sc, job, (float*)ctx->out.p + img * c0, nullptr);
+    rc = render_core(ctx, sc, job, (void*)((float*)ctx->out.p + img * c0), nullptr);
     c0 = c1;
   }
   if (rc == XRP_OK) {

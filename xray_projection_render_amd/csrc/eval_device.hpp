@@ -537,6 +537,24 @@ __device__ __forceinline__ bool clip_ray(const MarchParams& p, const Ray& r,
   return t_in <= t_out;  // false for NaN
 }
 
+// Stores pixel (view slot v, i, j) with transmittance exp(-T).  Mode 1
+// quantizes like main.go:485-498: uint16(val*0xffff) (Go converts through
+// int64 on amd64, keeping the low 16 bits) and the RGBA image keeps the
+// high byte; alpha is 0 where val >= 1 under transparency.
+__device__ __forceinline__ void store_pixel(const MarchParams& p, int v, int i,
+                                            int j, double T) {
+  const double val = exp(-T);
+  const size_t idx = (size_t)v * p.out_view_stride + (size_t)i * p.res + j;
+  if (p.out_mode == 0) {
+    p.out[idx] = (float)val;
+  } else {
+    unsigned v16 = (unsigned)((long long)(val * 65535.0) & 0xffff);
+    unsigned a16 = (p.transparency && !(val < 1.0)) ? 0u : 0xffffu;
+    p.out8[2 * idx] = (unsigned char)(v16 >> 8);
+    p.out8[2 * idx + 1] = (unsigned char)(a16 >> 8);
+  }
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -173,8 +173,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchParams p) {
         }
       }
     }
-    p.out[(size_t)blockIdx.y * p.out_view_stride + (size_t)i * p.res + j] =
-        (float)exp(-T);
+    store_pixel(p, blockIdx.y, i, j, T);
   }
 
   if (p.stats) {
@@ -187,6 +186,58 @@ __global__ __launch_bounds__(256) void march_kernel(MarchParams p) {
       if (d) atomicAdd(p.stats + ST_FINE, d);
     }
   }
+}
+
+// Volume export (main.go:549-635): density() at x = i/res*2-1 (the i/N
+// convention of computeVoxel, main.go:208-214), out[k*res*res + i*res + j].
+__global__ __launch_bounds__(256) void volume_density_kernel(MarchParams p, int res,
+                                                             double* out,
+                                                             unsigned long long* maxbits) {
+  const size_t n = (size_t)res * res * res;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (idx < n) {
+    size_t k = idx / ((size_t)res * res), i = (idx / res) % res, j = idx % res;
+    double x = (double)i / (double)res * 2.0 - 1.0;
+    double y = (double)j / (double)res * 2.0 - 1.0;
+    double z = (double)k / (double)res * 2.0 - 1.0;
+    v = density64(p, x, y, z) * p.dm;
+    out[idx] = v;
+  }
+  // max_val starts at 0 (main.go:619-624), so only positive values compete;
+  // their IEEE bit patterns order like unsigned integers.
+  unsigned long long b = v > 0.0 ? (unsigned long long)__double_as_longlong(v) : 0ull;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off, 64);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0 && b) atomicMax(maxbits, b);
+}
+
+// byte(v / max_val * 255): Go converts through int64 (CVTTSD2SQ on amd64)
+// and keeps the low byte; NaN and out-of-range values give 0.
+__global__ __launch_bounds__(256) void volume_quantize_kernel(const double* dens, size_t n,
+                                                              const unsigned long long* maxbits,
+                                                              unsigned char* out) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n) return;
+  double mx = __longlong_as_double((long long)*maxbits);
+  double q = dens[idx] / mx * 255;
+  long long iq = (q == q && q > -9.2e18 && q < 9.2e18) ? (long long)q : 0;
+  out[idx] = (unsigned char)(iq & 0xff);
+}
+
+int launch_volume(const MarchParams& p, int res, double* dens, unsigned long long* maxbits,
+                  unsigned char* out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  size_t n = (size_t)res * res * res;
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(volume_density_kernel, dim3(blocks), dim3(256), 0, s, p, res, dens,
+                     maxbits);
+  hipLaunchKernelGGL(volume_quantize_kernel, dim3(blocks), dim3(256), 0, s, dens, n, maxbits,
+                     out);
+  return (int)hipGetLastError();
 }
 
 int launch_march(const MarchParams& p, int precision, int integrator,

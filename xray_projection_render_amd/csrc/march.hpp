@@ -61,13 +61,23 @@ struct MarchParams {
   uint32_t vbytes;         // size of vgrid in bytes (buffer descriptor range)
   const float* vlin;       // linear [z][x][y] copy (LDS staging source)
   uint32_t vlin_bytes;
-  int32_t pad2_;
+  // Output: 0 = float32 transmittance in out; 1 = the reference PNG's
+  // 8-bit (gray, alpha) pair per pixel in out8 (main.go:481-498).
+  int32_t out_mode;
+  unsigned char* out8;
+  int32_t transparency;
+  int32_t pad3_;
 };
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with
 // the 4^3-brick copy as the gather fallback for oversized sub-volumes.
 enum VoxelLayout : int32_t { VOX_LINEAR = 0, VOX_BRICK4 = 1, VOX_LDS = 2 };
 int launch_voxel(const MarchParams& p, int layout, void* stream);
+
+// Volume export: densities (FP64) into dens, max into *maxbits (must start
+// at 0), bytes into out; res^3 entries each.
+int launch_volume(const MarchParams& p, int res, double* dens,
+                  unsigned long long* maxbits, unsigned char* out, void* stream);
 
 // Launch on the given stream; returns hipError_t as int.
 int launch_march(const MarchParams& p, int precision, int integrator,

@@ -341,6 +341,7 @@ HNode parse_object(const Value& d, const Num& N, Allowed al,
       n.nx = r3[0];
       n.ny = r3[1];
       n.nz = r3[2];
+      n.path = p;
     } else {
       for (const char* k : {"nx", "ny", "nz"})
         if (!N.is_int(d.get(k)))
@@ -823,6 +824,94 @@ void finalize_scene(Scene& s) {
   }
   s.world_box = w;
   s.min_feature = min_feature(s.root);
+}
+
+namespace {
+Value vnum(double x) {
+  Value v;
+  v.type = Value::Float;
+  v.f = x;
+  return v;
+}
+Value vint(long long x) {
+  Value v;
+  v.type = Value::Int;
+  v.i = x;
+  v.f = (double)x;
+  return v;
+}
+Value vstr(const std::string& s) {
+  Value v;
+  v.type = Value::String;
+  v.s = s;
+  return v;
+}
+Value vvec(const double* p) {
+  Value v;
+  v.type = Value::Arr;
+  v.a = std::make_shared<json::Array>();
+  for (int k = 0; k < 3; ++k) v.a->push_back(vnum(p[k]));
+  return v;
+}
+Value vmap(std::vector<std::pair<std::string, Value>> kv) {
+  std::sort(kv.begin(), kv.end(),
+            [](const auto& a, const auto& b) { return a.first < b.first; });
+  Value v;
+  v.type = Value::Obj;
+  v.o = std::make_shared<json::Object>(std::move(kv));
+  return v;
+}
+Value bounds_entries(const double* b, std::vector<std::pair<std::string, Value>>& kv) {
+  const char* k[6] = {"xmin", "xmax", "ymin", "ymax", "zmin", "zmax"};
+  for (int i = 0; i < 6; ++i) kv.emplace_back(k[i], vnum(b[i]));
+  return Value();
+}
+}  // namespace
+
+// ToMap of each object type (objects.go:33-40, 91-98, 139-146, 198-207,
+// 296-304, 370-379, 466-477, 523-534, 704-713, 973-981).
+Value to_map(const HNode& n) {
+  switch (n.kind) {
+    case K_SPHERE:
+      return vmap({{"type", vstr("sphere")}, {"center", vvec(n.c)},
+                   {"radius", vnum(n.a)}, {"rho", vnum(n.rho)}});
+    case K_BOX:
+      if (n.is_cube)
+        return vmap({{"type", vstr("cube")}, {"center", vvec(n.c)}, {"side", vnum(n.a)},
+                     {"rho", vnum(n.rho)}});
+      return vmap({{"type", vstr("box")}, {"center", vvec(n.c)}, {"sides", vvec(n.v)},
+                   {"rho", vnum(n.rho)}});
+    case K_PPED:
+      return vmap({{"type", vstr("parallelepiped")}, {"origin", vvec(n.c)},
+                   {"v0", vvec(n.pv)}, {"v1", vvec(n.pv + 3)}, {"v2", vvec(n.pv + 6)},
+                   {"rho", vnum(n.rho)}});
+    case K_CYL:
+      return vmap({{"type", vstr("cylinder")}, {"p0", vvec(n.c)}, {"p1", vvec(n.v)},
+                   {"radius", vnum(n.a)}, {"rho", vnum(n.rho)}});
+    case K_GYROID:
+      return vmap({{"type", vstr("gyroid")}, {"center", vvec(n.c)}, {"scale", vnum(n.a)},
+                   {"thickness", vnum(n.b)}, {"rho", vnum(n.rho)}});
+    case K_COLL: {
+      Value arr;
+      arr.type = Value::Arr;
+      arr.a = std::make_shared<json::Array>();
+      for (auto& k : n.kids) arr.a->push_back(to_map(k));
+      return vmap({{"type", vstr("object_collection")}, {"objects", arr}});
+    }
+    case K_TESS: {
+      std::vector<std::pair<std::string, Value>> uc{{"type", vstr("unit_cell")},
+                                                    {"objects", to_map(n.kids[0])}};
+      bounds_entries(n.ucb, uc);
+      std::vector<std::pair<std::string, Value>> t{{"type", vstr("tessellated_obj_coll")},
+                                                   {"uc", vmap(uc)}};
+      bounds_entries(n.bounds, t);
+      return vmap(t);
+    }
+    case K_VOXEL:
+      return vmap({{"type", vstr("voxel_grid")}, {"nx", vint(n.nx)}, {"ny", vint(n.ny)},
+                   {"nz", vint(n.nz)}, {"dtype", vstr("float64")}, {"path", vstr(n.path)}});
+  }
+  return Value();
 }
 
 void append_deformation(const Value& d, NumberMode numbers,

@@ -31,6 +31,7 @@ struct HNode {
   std::vector<HNode> kids;   // collection children (tess: the UC collection)
   std::shared_ptr<std::vector<float>> vox;  // [z][x][y]
   int nx = 0, ny = 0, nz = 0;
+  std::string path;          // voxel grid raw file as written in the scene
 };
 
 struct HDeform {
@@ -82,6 +83,10 @@ void append_deformation(const json::Value& d, NumberMode numbers,
 
 // Builds nodes/ddefs/boxes for a scene whose root/defs are filled.
 void finalize_scene(Scene& s);
+
+// The object as the reference's ToMap methods describe it (objects.go), for
+// object.json; maps are emitted with sorted keys like Go's encoder.
+json::Value to_map(const HNode& n);
 
 // FP32 position error model: returns the per-component bound on
 // |p32 - p64| of object-space positions and writes guard bands into

@@ -211,8 +211,7 @@ __global__ __launch_bounds__(256) void voxel_kernel(MarchParams p) {
     T += ((double)acc * dm) * ds;
   }
   if (valid)
-    p.out[(size_t)blockIdx.y * p.out_view_stride + (size_t)i * p.res + j] =
-        (float)exp(-T);
+    store_pixel(p, blockIdx.y, i, j, T);
   if (p.stats) {
     unsigned long long a = wave_sum(kb >= ka ? (unsigned long long)(kb - ka + 1) : 0ull);
     if (lane == 0 && a) atomicAdd(p.stats + ST_SAMPLES, a);
@@ -420,8 +419,7 @@ __global__ __launch_bounds__(64) void voxel_lds_kernel(MarchParams p) {
     T += ((double)acc * dm) * ds;
   }
   if (valid)
-    p.out[(size_t)blockIdx.y * p.out_view_stride + (size_t)i * p.res + j] =
-        (float)exp(-T);
+    store_pixel(p, blockIdx.y, i, j, T);
   if (p.stats) {
     unsigned long long a = wave_sum(kb >= ka ? (unsigned long long)(kb - ka + 1) : 0ull);
     if (lane == 0) {
