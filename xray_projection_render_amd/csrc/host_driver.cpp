@@ -283,6 +283,8 @@ void chunk(std::string& png, const char* type, const std::string& data) {
 }
 
 // pix: [i][j] (gray, alpha) pairs; PNG pixel (x=i, y=res-1-j), main.go:498.
+// image.RGBA holds alpha-premultiplied samples and PNG stores straight alpha,
+// so a fully transparent pixel is written as (0, 0, 0, 0).
 void write_png(const std::string& path, const unsigned char* pix, int res) {
   bool opaque = true;
   for (size_t k = 0; k < (size_t)res * res && opaque; ++k) opaque = pix[2 * k + 1] == 255;
@@ -294,9 +296,10 @@ void write_png(const std::string& path, const unsigned char* pix, int res) {
     int j = res - 1 - y;
     for (int x = 0; x < res; ++x) {
       const unsigned char* px = pix + 2 * ((size_t)x * res + j);
-      raw[o++] = px[0];
-      raw[o++] = px[0];
-      raw[o++] = px[0];
+      const unsigned char g = px[1] == 0 ? 0 : px[0];
+      raw[o++] = g;
+      raw[o++] = g;
+      raw[o++] = g;
       if (ch == 4) raw[o++] = px[1];
     }
   }
@@ -346,6 +349,19 @@ const Value* field(const Value& o, const char* name) {
   return nullptr;
 }
 
+// encoding/json's name for the kind of a mistyped JSON value
+std::string kind(const Value& v) {
+  switch (v.type) {
+    case Value::Bool: return "bool";
+    case Value::Int:
+    case Value::Float: return "number";
+    case Value::String: return "string";
+    case Value::Arr: return "array";
+    case Value::Obj: return "object";
+    default: return "null";
+  }
+}
+
 [[noreturn]] void type_error(const std::string& what, const char* name, const char* type) {
   throw Fatal("Failed to parse parameters: json: cannot unmarshal " + what +
               " into Go struct field RenderParams." + name + " of type " + type);
@@ -354,13 +370,13 @@ const Value* field(const Value& o, const char* name) {
 void get(const Value& o, const char* name, std::string& out) {
   const Value* v = field(o, name);
   if (!v || v->is_null()) return;
-  if (v->type != Value::String) type_error("value", name, "string");
+  if (v->type != Value::String) type_error(kind(*v), name, "string");
   out = v->s;
 }
 void get(const Value& o, const char* name, long long& out) {
   const Value* v = field(o, name);
   if (!v || v->is_null()) return;
-  if (!v->is_number()) type_error("value", name, "int");
+  if (!v->is_number()) type_error(kind(*v), name, "int");
   double d = v->number();
   if (d != std::floor(d)) type_error("number " + go_float(d), name, "int");
   out = (long long)d;
@@ -368,13 +384,13 @@ void get(const Value& o, const char* name, long long& out) {
 void get(const Value& o, const char* name, double& out) {
   const Value* v = field(o, name);
   if (!v || v->is_null()) return;
-  if (!v->is_number()) type_error("value", name, "float64");
+  if (!v->is_number()) type_error(kind(*v), name, "float64");
   out = v->number();
 }
 void get(const Value& o, const char* name, bool& out) {
   const Value* v = field(o, name);
   if (!v || v->is_null()) return;
-  if (v->type != Value::Bool) type_error("value", name, "bool");
+  if (v->type != Value::Bool) type_error(kind(*v), name, "bool");
   out = v->b;
 }
 
@@ -413,28 +429,30 @@ Params parse_params(const char* text) {
   if (const Value* ca = field(o, "camera_angles")) {
     if (ca->type == Value::Arr) {
       for (auto& a : *ca->a) {
-        if (a.type != Value::Obj) type_error("value", "camera_angles", "main.CameraAngle");
+        if (a.type != Value::Obj) type_error(kind(a), "camera_angles", "main.CameraAngle");
         double az = 0, po = 0;
         if (const Value* v = field(a, "azimuthal")) {
-          if (!v->is_number()) type_error("value", "camera_angles.azimuthal", "float64");
+          if (!v->is_number()) type_error(kind(*v), "camera_angles.azimuthal", "float64");
           az = v->number();
         }
         if (const Value* v = field(a, "polar")) {
-          if (!v->is_number()) type_error("value", "camera_angles.polar", "float64");
+          if (!v->is_number()) type_error(kind(*v), "camera_angles.polar", "float64");
           po = v->number();
         }
         p.angles.emplace_back(az, po);
       }
     } else if (!ca->is_null()) {
-      type_error("value", "camera_angles", "[]main.CameraAngle");
+      type_error(kind(*ca), "camera_angles", "[]main.CameraAngle");
     }
   }
   return p;
 }
 
+// A failed read is log.Fatal in the reference (main.go:57-59, 93-95), which
+// its API turns into the panic text "fatal: <msg>" (api.go:34-37).
 std::string read_file(const std::string& path, const char* what) {
   std::ifstream f(path, std::ios::binary);
-  if (!f) throw Fatal(std::string(what) + ": open " + path + ": no such file or directory");
+  if (!f) throw Fatal(std::string("fatal: ") + what);
   return std::string((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
 }
 
@@ -534,9 +552,14 @@ void render(const Params& p) {
     throw Fatal(std::string("Error unmarshalling scene file: ") + e.what());
   }
   SceneGuard sc;
-  // raw voxel paths resolve against the working directory, as os.ReadFile
-  check_rc(xrp_scene_create(obj_json.c_str(), def_json.empty() ? nullptr : def_json.c_str(),
-                            XRP_NUMBERS_YAML, nullptr, &sc.s));
+  // raw voxel paths resolve against the working directory, as os.ReadFile.
+  // A bad deformation is fatal in the reference (main.go:338-341); a bad
+  // object there renders a nil object and panics later, here it stops now.
+  int rc = xrp_scene_create(obj_json.c_str(), def_json.empty() ? nullptr : def_json.c_str(),
+                            XRP_NUMBERS_YAML, nullptr, &sc.s);
+  if (rc == XRP_ERR_DEFORMATION)
+    throw Fatal(std::string("fatal: Error loading deformation: ") + xrp_last_error());
+  if (rc != XRP_OK) throw Fatal(std::string("Error loading object: ") + xrp_last_error());
   mkdir_all(p.output_dir.empty() ? "." : p.output_dir);
 
   double ds = p.ds;
@@ -670,14 +693,19 @@ int xrp_scene_load_files(const char* object_path, const char* deformation_path,
   if (!out || !object_path) return xrp::set_error(XRP_ERR_NULL, "NULL argument");
   *out = nullptr;
   std::string obj_json, def_json;
+  const char* reading = object_path;
   try {
-    obj_json = typed_json(load_doc(object_path, read_file(object_path, "error reading object file")));
-    if (deformation_path && deformation_path[0])
-      def_json = typed_json(load_doc(
-          deformation_path, read_file(deformation_path, "error reading deformation file")));
-  } catch (const Fatal& e) {
-    return xrp::set_error(XRP_ERR_IO, e.what());
+    std::string text = read_file(object_path, "error reading object file");
+    reading = nullptr;
+    obj_json = typed_json(load_doc(object_path, text));
+    if (deformation_path && deformation_path[0]) {
+      reading = deformation_path;
+      text = read_file(deformation_path, "error reading deformation file");
+      reading = nullptr;
+      def_json = typed_json(load_doc(deformation_path, text));
+    }
   } catch (const std::exception& e) {
+    if (reading) return xrp::set_error(XRP_ERR_IO, std::string("cannot read ") + reading);
     return xrp::set_error(XRP_ERR_SCENE, std::string("Error unmarshalling: ") + e.what());
   }
   return xrp_scene_create(obj_json.c_str(), def_json.empty() ? nullptr : def_json.c_str(),
