@@ -84,10 +84,13 @@ def inbox_samples(w: W.Workload, lo: np.ndarray, hi: np.ndarray) -> int:
     return total
 
 
-def cpu_baseline(w: W.Workload, budget_s: float = 12.0) -> dict:
+def cpu_baseline(w: W.Workload, budget_s: float = 12.0):
     """FP64 oracle (all host cores, dynamic row scheduling) on a bounded
     sample of the workload: whole views in order until ~budget_s of work
-    (first a centre row band of view 0 to size the work)."""
+    (first a centre row band of view 0 to size the work).
+
+    Returns the baseline record and the oracle images it rendered as a list
+    of (view, row0, rows[view rows, res]) for the error check."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     if w.voxels is not None:
@@ -101,33 +104,57 @@ def cpu_baseline(w: W.Workload, budget_s: float = 12.0) -> dict:
     res = w.resolution
     az, po = w.angles[0]
     band = max(1, res // 32)
+    b0 = res // 2 - band // 2
     t = time.perf_counter()
-    sc.render([az], [po], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
-              nthreads=cores, rows=(res // 2 - band // 2, res // 2 - band // 2 + band))
+    img = sc.render([az], [po], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
+                    nthreads=cores, rows=(b0, b0 + band))
     probe = time.perf_counter() - t
+    images = [(0, b0, img[0, b0:b0 + band])]
     if probe * res / band > budget_s:  # even one view is too much: centre rows
         rows = max(1, int(band * budget_s / max(probe, 1e-9)))
         r0 = max(0, res // 2 - rows // 2)
         rows = min(rows, res - r0)
         t = time.perf_counter()
-        sc.render([az], [po], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
-                  nthreads=cores, rows=(r0, r0 + rows))
+        img = sc.render([az], [po], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
+                        nthreads=cores, rows=(r0, r0 + rows))
         spent = time.perf_counter() - t
+        images = [(0, r0, img[0, r0:r0 + rows])]
         pix, what = rows * res, f"view 0 rows {r0}..{r0 + rows - 1} of {res}"
     else:
         views, spent = 0, 0.0
+        images = []
         while spent < budget_s and views < len(w.angles):
             a, p = w.angles[views]
             t = time.perf_counter()
-            sc.render([a], [p], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
-                      nthreads=cores)
+            img = sc.render([a], [p], res, w.ds, R=w.R, fov=w.fov, integrator=w.integrator,
+                            nthreads=cores)
             spent += time.perf_counter() - t
+            images.append((views, 0, img[0]))
             views += 1
         pix, what = views * res * res, f"views 0..{views - 1} of {len(w.angles)}, full images"
-    return {"value": pix / spent / (512 * 512), "unit": "512² projections/s",
-            "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle (oracle/xrp_oracle.c, full [R-1.74, R+1.74] segment, "
-                      f"no clipping), {w.name}: {what}, {spent:.1f} s"}
+    rec = {"value": pix / spent / (512 * 512), "unit": "512² projections/s",
+           "cores": cores, "kind": "port",
+           "sample": f"FP64 oracle (oracle/xrp_oracle.c, full [R-1.74, R+1.74] segment, "
+                     f"no clipping), {w.name}: {what}, {spent:.1f} s"}
+    return rec, images
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` as recorded by a
+    rocprofv3 --pmc summary under profiles/ (a line
+    `traffic_bytes_per_launch: <kernel> | <workload> | <bytes>`), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for name in sorted(os.listdir(d)):
+        with open(os.path.join(d, name), errors="replace") as fh:
+            for line in fh:
+                if not line.startswith("traffic_bytes_per_launch:"):
+                    continue
+                parts = [x.strip() for x in line.split(":", 1)[1].split("|")]
+                if len(parts) == 3 and parts[0] == kernel and parts[1] == workload:
+                    return float(parts[2])
+    return None
 
 
 def main() -> int:
@@ -138,7 +165,19 @@ def main() -> int:
     ap.add_argument("--workload", default="voxel")
     ap.add_argument("--precision", default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device 3 times: the reference "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(256 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": 3 * a.numel() * 4}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this renderer runs on one GPU; views shard across "
                           "GPUs with no exchange, but only --gpus 1 is measured"}))
@@ -182,9 +221,12 @@ def main() -> int:
         roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None}
     if w.voxels is not None and prec != "fp64" and w.integrator == "simple":
-        layout = os.environ.get("XRP_VOXEL_LAYOUT", "brick4")
-        roof["kernel"] = {"linear": "voxel_kernel<VOX_LINEAR>",
-                          "brick4": "voxel_kernel<VOX_BRICK4>"}.get(layout, "voxel_lds_kernel")
+        layout = os.environ.get("XRP_VOXEL_LAYOUT", "cached")
+        roof["kernel"] = {"linear": "voxel_kernel<VOX_LINEAR,CACHE_NONE>",
+                          "brick4": "voxel_kernel<VOX_BRICK4,CACHE_NONE>",
+                          "cached": "voxel_kernel<VOX_BRICK4,CACHE_CELL>",
+                          "linear_cached": "voxel_kernel<VOX_LINEAR,CACHE_CELL>",
+                          }.get(layout, "voxel_lds_kernel")
     else:
         roof["kernel"] = "march_kernel<%s,%s>" % (
             "PREC_FP64" if prec == "fp64" else "PREC_GUARDED",
@@ -211,8 +253,16 @@ def main() -> int:
                                             "fallback_chunks")},
         "roofline": roof,
     }
+    roof["traffic"] = measured_traffic(roof["kernel"], w.name)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w)
+        line["cpu_baseline"], images = cpu_baseline(w)
+        # the same pixels from the last timed GPU render
+        err, n = 0.0, 0
+        for v, r0, ref in images:
+            got = out[v, r0:r0 + ref.shape[0]].double().cpu().numpy()
+            err = max(err, float(np.max(np.abs(got - ref))))
+            n += ref.size
+        line["error_vs_oracle"] = {"max_abs": err, "pixels": n, "tolerance": 1e-5}
     print(json.dumps(line))
     scene.close()
     ctx.close()

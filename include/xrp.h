@@ -43,7 +43,10 @@ typedef enum {
 typedef enum { XRP_INTEGRATE_SIMPLE = 0, XRP_INTEGRATE_HIERARCHICAL = 1 } xrp_integrator;
 
 /* Precision of the density evaluation.
- *  AUTO    - FP64 for scenes holding a voxel grid, GUARDED otherwise.
+ *  AUTO    - a voxel-grid root under linear deformations with the simple
+ *            integrator takes the FP32 voxel kernel (exact FP64 sample
+ *            range per ray, FP32 trilinear, Kahan-compensated sum); other
+ *            scenes holding a voxel grid FP64; everything else GUARDED.
  *  GUARDED - FP32 membership tests with guard bands; samples inside a band
  *            are re-evaluated in FP64.  Densities and the integral are FP64.
  *  FP64    - every sample evaluated in FP64 in the reference's order. */

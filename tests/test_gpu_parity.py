@@ -131,7 +131,10 @@ def blob256():
     return W.blob_grid(256)
 
 
-@pytest.mark.parametrize("layout", ["lds", "brick4", "linear"])
+VOXEL_LAYOUTS = ["lds", "brick4", "linear", "cached", "linear_cached"]
+
+
+@pytest.mark.parametrize("layout", VOXEL_LAYOUTS)
 def test_voxel_full_config(ctx, blob256, layout, monkeypatch):
     """The 256^3 blob grid at 512^2 through the dedicated voxel kernel,
     8 of the benchmark's 64 views, against the FP64 oracle."""
@@ -144,9 +147,25 @@ def test_voxel_full_config(ctx, blob256, layout, monkeypatch):
     assert ref.min() < 0.6  # the blobs absorb
 
 
-def test_voxel_boundary_ties(ctx):
+def test_voxel_layouts_bit_identical(ctx, blob256, monkeypatch):
+    """Brick order and the per-lane corner cache change where corners are
+    read from, not the arithmetic: every gather layout gives the same bits."""
+    w = W.voxel(grid=blob256, n_views=6)
+    sc = xrp_scene(None, w.deformation, voxels=blob256)
+    imgs = {}
+    for layout in ["brick4", "linear", "cached", "linear_cached"]:
+        monkeypatch.setenv("XRP_VOXEL_LAYOUT", layout)
+        imgs[layout], _ = X.render(ctx, sc, w.angles, w.resolution, w.ds,
+                                   integrator="simple")
+    for layout, img in imgs.items():
+        assert np.array_equal(img, imgs["brick4"]), layout
+
+
+@pytest.mark.parametrize("layout", ["brick4", "cached"])
+def test_voxel_boundary_ties(ctx, layout, monkeypatch):
     """Axis-aligned views put samples on the grid faces x = +-1 exactly;
     the per-ray FP64 range must count them as the reference does."""
+    monkeypatch.setenv("XRP_VOXEL_LAYOUT", layout)
     grid = np.ones((9, 9, 9), dtype=np.float32)
     for ds in (0.025, 0.0125, 0.01):
         err, st, img, ref = _compare(ctx, None, [(0.0, 90.0), (90.0, 90.0), (180.0, 90.0)],

@@ -71,7 +71,17 @@ struct MarchParams {
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with
 // the 4^3-brick copy as the gather fallback for oversized sub-volumes.
-enum VoxelLayout : int32_t { VOX_LINEAR = 0, VOX_BRICK4 = 1, VOX_LDS = 2 };
+// *_CACHED: per-lane register cache of the last cell's 8 corners.
+enum VoxelLayout : int32_t {
+  VOX_LINEAR = 0,
+  VOX_BRICK4 = 1,
+  VOX_LDS = 2,
+  VOX_BRICK4_CACHED = 3,
+  VOX_LINEAR_CACHED = 4
+};
+inline bool voxel_layout_linear(int layout) {
+  return layout == VOX_LINEAR || layout == VOX_LINEAR_CACHED;
+}
 int launch_voxel(const MarchParams& p, int layout, void* stream);
 
 // Volume export: densities (FP64) into dens, max into *maxbits (must start

@@ -25,50 +25,74 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t grid_rsrc(const float* g,
                                            0x00020000);
 }
 
+// Element offsets of the two x, y and z planes of cell (x0, y0, z0), upper
+// indices clamped to N-1 as objects.go:811-830 does.
+struct CellOffsets {
+  unsigned x[2], y[2], z[2];
+};
+
 template <int LAYOUT>
-__device__ __forceinline__ float trilinear(const MarchParams& p,
-                                           __amdgpu_buffer_rsrc_t rs, float gx,
-                                           float gy, float gz) {
+__device__ __forceinline__ CellOffsets cell_offsets(const MarchParams& p, int x0, int y0,
+                                                    int z0) {
   const int nx = p.vn[0], ny = p.vn[1], nz = p.vn[2];
-  int x0 = min((int)gx, nx - 1), y0 = min((int)gy, ny - 1),
-      z0 = min((int)gz, nz - 1);
-  int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1),
-      z1 = min(z0 + 1, nz - 1);
-  float wx = gx - (float)x0, wy = gy - (float)y0, wz = gz - (float)z0;
-  unsigned ox0, ox1, oy0, oy1, oz0, oz1;
+  const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1), z1 = min(z0 + 1, nz - 1);
+  CellOffsets o;
   if constexpr (LAYOUT == VOX_LINEAR) {
-    oz0 = (unsigned)z0 * (unsigned)(nx * ny);
-    oz1 = (unsigned)z1 * (unsigned)(nx * ny);
-    ox0 = (unsigned)x0 * (unsigned)ny;
-    ox1 = (unsigned)x1 * (unsigned)ny;
-    oy0 = (unsigned)y0;
-    oy1 = (unsigned)y1;
+    o.z[0] = (unsigned)z0 * (unsigned)(nx * ny);
+    o.z[1] = (unsigned)z1 * (unsigned)(nx * ny);
+    o.x[0] = (unsigned)x0 * (unsigned)ny;
+    o.x[1] = (unsigned)x1 * (unsigned)ny;
+    o.y[0] = (unsigned)y0;
+    o.y[1] = (unsigned)y1;
   } else {
     // 4x4x4 bricks ordered (bz, bx, by); inside a brick (lz, lx, ly).
     const unsigned by_n = (unsigned)p.vbricks[1];
     const unsigned sx = by_n * 64u, sz = (unsigned)p.vbricks[0] * by_n * 64u;
-    oz0 = (unsigned)(z0 >> 2) * sz + (unsigned)(z0 & 3) * 16u;
-    oz1 = (unsigned)(z1 >> 2) * sz + (unsigned)(z1 & 3) * 16u;
-    ox0 = (unsigned)(x0 >> 2) * sx + (unsigned)(x0 & 3) * 4u;
-    ox1 = (unsigned)(x1 >> 2) * sx + (unsigned)(x1 & 3) * 4u;
-    oy0 = (unsigned)(y0 >> 2) * 64u + (unsigned)(y0 & 3);
-    oy1 = (unsigned)(y1 >> 2) * 64u + (unsigned)(y1 & 3);
+    o.z[0] = (unsigned)(z0 >> 2) * sz + (unsigned)(z0 & 3) * 16u;
+    o.z[1] = (unsigned)(z1 >> 2) * sz + (unsigned)(z1 & 3) * 16u;
+    o.x[0] = (unsigned)(x0 >> 2) * sx + (unsigned)(x0 & 3) * 4u;
+    o.x[1] = (unsigned)(x1 >> 2) * sx + (unsigned)(x1 & 3) * 4u;
+    o.y[0] = (unsigned)(y0 >> 2) * 64u + (unsigned)(y0 & 3);
+    o.y[1] = (unsigned)(y1 >> 2) * 64u + (unsigned)(y1 & 3);
   }
-  // the builtin returns the 32 raw bits as an unsigned int
-#define XRP_LD(o) \
-  __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((o) * 4u), 0, 0))
-  float v000 = XRP_LD(oz0 + ox0 + oy0), v001 = XRP_LD(oz1 + ox0 + oy0);
-  float v010 = XRP_LD(oz0 + ox0 + oy1), v011 = XRP_LD(oz1 + ox0 + oy1);
-  float v100 = XRP_LD(oz0 + ox1 + oy0), v101 = XRP_LD(oz1 + ox1 + oy0);
-  float v110 = XRP_LD(oz0 + ox1 + oy1), v111 = XRP_LD(oz1 + ox1 + oy1);
-#undef XRP_LD
-  float v00 = fmaf(wz, v001 - v000, v000);
-  float v01 = fmaf(wz, v011 - v010, v010);
-  float v10 = fmaf(wz, v101 - v100, v100);
-  float v11 = fmaf(wz, v111 - v110, v110);
+  return o;
+}
+
+// the builtin returns the 32 raw bits as an unsigned int
+__device__ __forceinline__ float grid_load(__amdgpu_buffer_rsrc_t rs, unsigned elem) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(elem * 4u), 0, 0));
+}
+
+// c[4*dx + 2*dy + dz] is the voxel at (x0+dx, y0+dy, z0+dz).
+template <int LAYOUT>
+__device__ __forceinline__ void load_cell(const MarchParams& p, __amdgpu_buffer_rsrc_t rs,
+                                          int x0, int y0, int z0, float (&c)[8]) {
+  const CellOffsets o = cell_offsets<LAYOUT>(p, x0, y0, z0);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) c[q] = grid_load(rs, o.z[q & 1] + o.x[q >> 2] + o.y[(q >> 1) & 1]);
+}
+
+// Lerp order z, then y, then x (objects.go:845-851).
+__device__ __forceinline__ float lerp_cell(const float (&c)[8], float wx, float wy,
+                                           float wz) {
+  float v00 = fmaf(wz, c[1] - c[0], c[0]);
+  float v01 = fmaf(wz, c[3] - c[2], c[2]);
+  float v10 = fmaf(wz, c[5] - c[4], c[4]);
+  float v11 = fmaf(wz, c[7] - c[6], c[6]);
   float v0 = fmaf(wy, v01 - v00, v00);
   float v1 = fmaf(wy, v11 - v10, v10);
   return fmaf(wx, v1 - v0, v0);
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ float trilinear(const MarchParams& p,
+                                           __amdgpu_buffer_rsrc_t rs, float gx,
+                                           float gy, float gz) {
+  int x0 = min((int)gx, p.vn[0] - 1), y0 = min((int)gy, p.vn[1] - 1),
+      z0 = min((int)gz, p.vn[2] - 1);
+  float c[8];
+  load_cell<LAYOUT>(p, rs, x0, y0, z0, c);
+  return lerp_cell(c, gx - (float)x0, gy - (float)y0, gz - (float)z0);
 }
 
 // Exact FP64 membership of sample k in the grid's closed [-1, 1]^3 support
@@ -127,8 +151,15 @@ __device__ __forceinline__ bool exact_range(const MarchParams& p, const Ray& r,
   return ka <= kb && ka <= last;
 }
 
-template <int LAYOUT>
-__global__ __launch_bounds__(256) void voxel_kernel(MarchParams p) {
+// CACHE_CELL keeps the last cell's 8 corners in registers and reloads them
+// only when the cell changes (a step of ds moves a ray through the 256^3
+// benchmark grid ~0.4 voxel, so most samples reuse the cell).  The values,
+// and so the result, are the same as without the cache.  (Reusing the
+// shared face when the cell moves by one along one axis, reading only four
+// corners, measured 3x slower: the four divergent paths serialize.)
+enum VoxelCache { CACHE_NONE = 0, CACHE_CELL = 1 };
+template <int LAYOUT, int CACHE>
+__device__ __forceinline__ void voxel_body(const MarchParams& p) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int tiles_j = (p.res + 15) >> 4;
@@ -188,6 +219,10 @@ __global__ __launch_bounds__(256) void voxel_kernel(MarchParams p) {
   const float ds32 = (float)p.ds;
   const double ds = p.ds, dm = p.dm;
   const __amdgpu_buffer_rsrc_t rs = grid_rsrc(p.vgrid, p.vbytes);
+  int cx = -1, cy = -1, cz = -1;  // cached cell (CACHED only)
+  float cell[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) cell[q] = 0.0f;
   for (int k0 = wk0; k0 <= wk1; k0 += kChunk) {
     double sa = p.s_table[k0];
     float a0 = (float)(A[0] + B[0] * sa), a1 = (float)(A[1] + B[1] * sa),
@@ -198,10 +233,22 @@ __global__ __launch_bounds__(256) void voxel_kernel(MarchParams p) {
     for (int jj = 0; jj < n; ++jj) {
       if (jj >= jlo && jj <= jhi) {
         float sj = (float)jj * ds32;
-        float gx = fmaf(b0, sj, a0), gy = fmaf(b1, sj, a1),
-              gz = fmaf(b2, sj, a2);
-        float rho = trilinear<LAYOUT>(p, rs, fmaxf(gx, 0.0f), fmaxf(gy, 0.0f),
-                                      fmaxf(gz, 0.0f));
+        float gx = fmaxf(fmaf(b0, sj, a0), 0.0f), gy = fmaxf(fmaf(b1, sj, a1), 0.0f),
+              gz = fmaxf(fmaf(b2, sj, a2), 0.0f);
+        float rho;
+        if constexpr (CACHE != CACHE_NONE) {
+          int x0 = min((int)gx, p.vn[0] - 1), y0 = min((int)gy, p.vn[1] - 1),
+              z0 = min((int)gz, p.vn[2] - 1);
+          if (x0 != cx || y0 != cy || z0 != cz) {
+            load_cell<LAYOUT>(p, rs, x0, y0, z0, cell);
+            cx = x0;
+            cy = y0;
+            cz = z0;
+          }
+          rho = lerp_cell(cell, gx - (float)x0, gy - (float)y0, gz - (float)z0);
+        } else {
+          rho = trilinear<LAYOUT>(p, rs, gx, gy, gz);
+        }
         float y = rho - comp;
         float tt = acc + y;
         comp = (tt - acc) - y;
@@ -216,6 +263,13 @@ __global__ __launch_bounds__(256) void voxel_kernel(MarchParams p) {
     unsigned long long a = wave_sum(kb >= ka ? (unsigned long long)(kb - ka + 1) : 0ull);
     if (lane == 0 && a) atomicAdd(p.stats + ST_SAMPLES, a);
   }
+}
+
+// 8 waves per SIMD: <= 64 VGPRs, which every variant meets without spills.
+template <int LAYOUT, int CACHE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void voxel_kernel(
+    MarchParams p) {
+  voxel_body<LAYOUT, CACHE>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,10 +497,19 @@ int launch_voxel(const MarchParams& p, int layout, void* stream) {
   int tiles = (p.res + 15) / 16;
   dim3 grid(tiles * tiles, p.n_views);
   hipStream_t s = (hipStream_t)stream;
-  if (layout == VOX_BRICK4)
-    hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4>), grid, block, 0, s, p);
-  else
-    hipLaunchKernelGGL((voxel_kernel<VOX_LINEAR>), grid, block, 0, s, p);
+  switch (layout) {
+    case VOX_BRICK4:
+      hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4, CACHE_NONE>), grid, block, 0, s, p);
+      break;
+    case VOX_BRICK4_CACHED:
+      hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4, CACHE_CELL>), grid, block, 0, s, p);
+      break;
+    case VOX_LINEAR_CACHED:
+      hipLaunchKernelGGL((voxel_kernel<VOX_LINEAR, CACHE_CELL>), grid, block, 0, s, p);
+      break;
+    default:
+      hipLaunchKernelGGL((voxel_kernel<VOX_LINEAR, CACHE_NONE>), grid, block, 0, s, p);
+  }
   return (int)hipGetLastError();
 }
 
