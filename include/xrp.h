@@ -196,6 +196,67 @@ int xrp_render_volume_projections(xrp_context *ctx, const float *volume, int nx,
 int xrp_export_volume(xrp_context *ctx, xrp_scene *scene, int res,
                       double density_multiplier, uint8_t *out);
 
+/* ---- batched rendering ------------------------------------------------------
+ * Many scenes, each with its own views and parameters, rendered by one
+ * persistent launch per (precision, integrator) group over an atomic queue
+ * of (scene, view, tile) items, the sequence captured once in a hipGraph
+ * and replayed by xrp_batch_run.  It batches the per-view loop of render()
+ * (main.go:427-523) across scenes; the reference renders one scene per
+ * process.  Images are float32 transmittance, [view][i][j] per item, kept
+ * in device memory owned by the batch. */
+typedef struct {
+  const xrp_scene *scene;
+  const xrp_view_angle *views;
+  int32_t n_views;
+  int32_t reserved;
+  xrp_render_params params; /* output_format must be XRP_OUTPUT_F32 */
+} xrp_batch_item;
+
+/* Work item of the queue: wave_rows x wave_cols blocks of 8x8 pixels that
+ * one wave marches in sequence (at most 4 blocks); groups_per_cu persistent
+ * 256-thread workgroups are launched per compute unit. */
+typedef struct {
+  int32_t wave_rows;
+  int32_t wave_cols;
+  int32_t groups_per_cu;
+  int32_t reserved;
+} xrp_tile_shape;
+
+typedef struct {
+  double kernel_ms;     /* graph replay, HIP events on the context stream */
+  uint64_t rays;
+  uint64_t samples;
+  uint64_t reevaluated;
+  uint64_t tiles;
+  int32_t launches;     /* persistent launches in the graph */
+  int32_t reserved;
+} xrp_batch_stats;
+
+typedef struct xrp_batch xrp_batch;
+
+/* Plans every item (precision, guard bands, sample tables), uploads the
+ * scenes, views and tables, allocates the images and captures the graph
+ * with the default tile shape {1, 1, 2}.  The scenes must outlive the batch. */
+int xrp_batch_create(xrp_context *ctx, const xrp_batch_item *items, int n_items,
+                     xrp_batch **out);
+/* Re-tiles the queue and re-captures the graph. */
+int xrp_batch_set_shape(xrp_batch *batch, const xrp_tile_shape *shape);
+/* Autotuner: times each candidate shape over `repeats` replays, keeps the
+ * fastest and records it in a process-wide table keyed by the batch's
+ * signature (and in the file named by $XRP_TUNE_CACHE when set), so a batch
+ * with the same signature starts from the recorded shape. */
+int xrp_batch_tune(xrp_batch *batch, int repeats, xrp_tile_shape *chosen);
+/* Replays the graph and waits for it. */
+int xrp_batch_run(xrp_batch *batch, xrp_batch_stats *stats);
+/* Device pointer of item's images (valid until xrp_batch_destroy). */
+int xrp_batch_output(const xrp_batch *batch, int item, const float **device_images);
+/* Copies item's images to host memory (n_views * res * res floats). */
+int xrp_batch_read(xrp_batch *batch, int item, float *host_images);
+/* Per item, milliseconds from the batch's first tile start to the item's
+ * last tile end in the last run (n_items values). */
+int xrp_batch_latencies(const xrp_batch *batch, double *ms);
+void xrp_batch_destroy(xrp_batch *batch);
+
 /* ---- reference JSON API (api.go:82-180) ----------------------------------- */
 /* Renders as the reference's RenderProjections does (PNG frames,
  * transforms.json, object.json) on the GPU.  Returns a malloc'd JSON

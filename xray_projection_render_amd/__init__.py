@@ -6,10 +6,10 @@ Beer-Lambert march over the primitive tree and deformation fields, and the
 trilinear voxel sampler run as gfx950 kernels in libxrp.so.  XRayRenderer
 keeps the reference's Python interface.
 """
-from ._lib import (Context, Scene, XrpError, camera_from_angles, export_volume, lib,
+from ._lib import (Batch, Context, Scene, XrpError, camera_from_angles, export_volume, lib,
                    render, render_device, render_volume_projections, sample_count)
 from .xray_renderer import XRayRenderer
 
-__all__ = ["Context", "Scene", "XRayRenderer", "XrpError", "camera_from_angles",
+__all__ = ["Batch", "Context", "Scene", "XRayRenderer", "XrpError", "camera_from_angles",
            "export_volume", "lib", "render", "render_device", "render_volume_projections",
            "sample_count"]

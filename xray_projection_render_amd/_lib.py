@@ -81,6 +81,30 @@ class Camera(ctypes.Structure):
 _lib = None
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("views", ctypes.POINTER(ViewAngle)),
+                ("n_views", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("params", RenderParams)]
+
+
+class TileShape(ctypes.Structure):
+    _fields_ = [("wave_rows", ctypes.c_int32), ("wave_cols", ctypes.c_int32),
+                ("groups_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_tuple(self) -> Tuple[int, int, int]:
+        return (self.wave_rows, self.wave_cols, self.groups_per_cu)
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("rays", ctypes.c_uint64),
+                ("samples", ctypes.c_uint64), ("reevaluated", ctypes.c_uint64),
+                ("tiles", ctypes.c_uint64), ("launches", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+    def as_dict(self) -> Dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
@@ -119,6 +143,14 @@ def lib() -> ctypes.CDLL:
                                               D, D, FP]),
         "xrp_export_volume": (I, [P, P, I, D, ctypes.POINTER(ctypes.c_uint8)]),
         "xrp_scene_load_files": (I, [ctypes.c_char_p, ctypes.c_char_p, PP]),
+        "xrp_batch_create": (I, [P, ctypes.POINTER(BatchItem), I, PP]),
+        "xrp_batch_set_shape": (I, [P, ctypes.POINTER(TileShape)]),
+        "xrp_batch_tune": (I, [P, I, ctypes.POINTER(TileShape)]),
+        "xrp_batch_run": (I, [P, ctypes.POINTER(BatchStats)]),
+        "xrp_batch_output": (I, [P, I, ctypes.POINTER(FP)]),
+        "xrp_batch_read": (I, [P, I, FP]),
+        "xrp_batch_latencies": (I, [P, DP]),
+        "xrp_batch_destroy": (None, [P]),
         "RenderProjections": (ctypes.POINTER(ctypes.c_char), [ctypes.c_char_p]),
         "FreeString": (None, [ctypes.POINTER(ctypes.c_char)]),
     }
@@ -351,3 +383,69 @@ def render_volume_projections(ctx: Context, volume: np.ndarray,
         len(cams), image_res, ds, flat_field,
         out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
     return out
+
+
+class Batch:
+    """Many scenes rendered by one graph-captured persistent launch
+    (xrp_batch_*).  items: (scene, angles, resolution, ds, render kwargs)."""
+
+    def __init__(self, ctx: Context, items: Sequence[Tuple]):
+        self._keep = []
+        arr = (BatchItem * len(items))()
+        self.shapes = []
+        for k, it in enumerate(items):
+            scene, angles, res, ds = it[:4]
+            kw = it[4] if len(it) > 4 else {}
+            views = _views(angles)
+            self._keep.append((scene, views))
+            arr[k].scene = scene.ptr
+            arr[k].views = ctypes.cast(views, ctypes.POINTER(ViewAngle))
+            arr[k].n_views = len(angles)
+            arr[k].params = make_params(res, ds, **kw)
+            self.shapes.append((len(angles), res, res))
+        ptr = ctypes.c_void_p()
+        check(lib().xrp_batch_create(ctx.ptr, arr, len(items), ctypes.byref(ptr)))
+        self.ptr = ptr
+        self.ctx = ctx
+
+    def set_shape(self, wave_rows: int, wave_cols: int, groups_per_cu: int) -> None:
+        t = TileShape(wave_rows, wave_cols, groups_per_cu, 0)
+        check(lib().xrp_batch_set_shape(self.ptr, ctypes.byref(t)))
+
+    def tune(self, repeats: int = 2) -> Tuple[int, int, int]:
+        t = TileShape()
+        check(lib().xrp_batch_tune(self.ptr, repeats, ctypes.byref(t)))
+        return t.as_tuple()
+
+    def run(self) -> Dict:
+        st = BatchStats()
+        check(lib().xrp_batch_run(self.ptr, ctypes.byref(st)))
+        return st.as_dict()
+
+    def output_ptr(self, item: int) -> int:
+        fp = ctypes.POINTER(ctypes.c_float)()
+        check(lib().xrp_batch_output(self.ptr, item, ctypes.byref(fp)))
+        return ctypes.cast(fp, ctypes.c_void_p).value
+
+    def read(self, item: int) -> np.ndarray:
+        out = np.empty(self.shapes[item], dtype=np.float32)
+        check(lib().xrp_batch_read(self.ptr, item,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def latencies(self) -> np.ndarray:
+        out = np.zeros(len(self.shapes), dtype=np.float64)
+        check(lib().xrp_batch_latencies(self.ptr, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().xrp_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -32,6 +32,7 @@ DEVICE = ["-x", "hip", "--offload-arch=" + ARCH,
 SOURCES = [
     ("march.hip", True),
     ("voxel.hip", True),
+    ("batch.hip", True),
     ("api.cpp", False),
     ("scene.cpp", False),
     ("camera.cpp", False),

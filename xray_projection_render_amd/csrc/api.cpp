@@ -414,19 +414,32 @@ size_t mp_bricks(const xrp::HNode& n) {
   return (size_t)((n.nx + 3) / 4) * ((n.ny + 3) / 4) * ((n.nz + 3) / 4);
 }
 
-int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
-                void* out_dev, xrp_render_stats* st) {
+// Everything a launch of one scene needs, computed on the host: the
+// resolved precision, the sample table, the node table with guard bands and
+// voxel pointers, and the launch parameters.  Device pointers for nodes,
+// deformations, views, the sample table, stats and the output are left for
+// the caller to fill in.
+struct LaunchPlan {
+  xrp::MarchParams mp;
+  std::vector<xrp::DevNode> nodes;
+  std::vector<double> table;
+  int precision = XRP_PRECISION_GUARDED;
+  bool vox_kernel = false;
+  int layout = xrp::VOX_BRICK4_CACHED;
+  double delta = 0.0;
+};
+
+int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& pl,
+             bool allow_voxel_kernel = true) {
   using namespace xrp;
   Scene& s = *sc->s;
-  const int nv = (int)job.views.size();
   const int res = job.res;
-  HIPCHK(hipSetDevice(ctx->device));
 
   // Voxel-root scenes under linear deformations take the dedicated voxel
-  // kernel (FP32 grid stepping, FP64 re-evaluation at the grid faces).
+  // kernel (FP32 grid stepping, exact FP64 sample range per ray).
   double vM[9], vt[3];
   const bool vox_kernel =
-      s.root.kind == K_VOXEL && compose_linear(s.defs, vM, vt) &&
+      allow_voxel_kernel && s.root.kind == K_VOXEL && compose_linear(s.defs, vM, vt) &&
       job.integrator == XRP_INTEGRATE_SIMPLE && !job.audit &&
       job.precision != XRP_PRECISION_FP64 &&
       (size_t)mp_bricks(s.root) * 64 < ((size_t)1 << 30);
@@ -434,25 +447,10 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   if (precision == XRP_PRECISION_AUTO)
     precision = (s.has_voxels && !vox_kernel) ? XRP_PRECISION_FP64 : XRP_PRECISION_GUARDED;
 
-  // Sample table (cached per context).
   const double smin = job.R - kCubeHalfDiagonal, smax = job.R + kCubeHalfDiagonal;
-  if (ctx->table_mode != job.integrator || ctx->table_smin != smin ||
-      ctx->table_smax != smax || ctx->table_ds != job.ds) {
-    ctx->table_host = job.integrator == XRP_INTEGRATE_HIERARCHICAL
-                          ? coarse_samples(smin, smax, job.ds)
-                          : simple_samples(smin, smax, job.ds);
-    HIPCHK(ctx->table.ensure(std::max<size_t>(1, ctx->table_host.size()) *
-                             sizeof(double)));
-    if (!ctx->table_host.empty())
-      HIPCHK(hipMemcpy(ctx->table.p, ctx->table_host.data(),
-                       ctx->table_host.size() * sizeof(double),
-                       hipMemcpyHostToDevice));
-    ctx->table_mode = job.integrator;
-    ctx->table_smin = smin;
-    ctx->table_smax = smax;
-    ctx->table_ds = job.ds;
-  }
-  const int K = (int)ctx->table_host.size();
+  pl.table = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? coarse_samples(smin, smax, job.ds)
+                                                          : simple_samples(smin, smax, job.ds);
+  const int K = (int)pl.table.size();
 
   // Clip box and FP32 error model.
   const Box3& wb = s.world_box;
@@ -490,33 +488,13 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     int rc = upload_voxels(ctx, sc, vptr);
     if (rc) return rc;
   }
+  pl.nodes = s.nodes;
   size_t vi = 0;
-  for (auto& n : s.nodes)
+  for (auto& n : pl.nodes)
     if (n.kind == K_VOXEL) n.vox = (const float*)vptr[vi++];
 
-  HIPCHK(ctx->nodes.ensure(s.nodes.size() * sizeof(DevNode)));
-  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, s.ddefs.size()) * sizeof(DevDeform)));
-  HIPCHK(ctx->views.ensure(nv * sizeof(DevView)));
-  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
-  HIPCHK(hipMemcpyAsync(ctx->nodes.p, s.nodes.data(),
-                        s.nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice,
-                        ctx->stream));
-  if (!s.ddefs.empty())
-    HIPCHK(hipMemcpyAsync(ctx->defs.p, s.ddefs.data(),
-                          s.ddefs.size() * sizeof(DevDeform),
-                          hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemcpyAsync(ctx->views.p, job.views.data(), nv * sizeof(DevView),
-                        hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long),
-                        ctx->stream));
-
-  MarchParams mp;
+  MarchParams& mp = pl.mp;
   std::memset(&mp, 0, sizeof mp);
-  mp.nodes = (const DevNode*)ctx->nodes.p;
-  mp.defs = (const DevDeform*)ctx->defs.p;
-  mp.views = (const DevView*)ctx->views.p;
-  mp.s_table = (const double*)ctx->table.p;
-  mp.stats = (unsigned long long*)ctx->stats.p;
   mp.n_defs = (int)s.ddefs.size();
   mp.res = res;
   mp.K = K;
@@ -537,6 +515,7 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   mp.out_view_stride = res * res;
   mp.out_mode = job.output;
   mp.transparency = job.transparency;
+  mp.n_views = (int)job.views.size();
   // 4^3-brick gathers with the per-lane cell cache measured fastest on the
   // 256^3 voxel workload (4.1k projections/s; 3.0k uncached bricks, 2.8k
   // cached linear, 2.2k LDS-staged, 1.6k linear); the others stay
@@ -575,6 +554,68 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     // FP32 anchor rounding (u*|G|) plus the FP32 step (4u * |B| * chunk*ds).
     mp.vguard = (float)(4.0 * (u * (N + 4.0 * L * 0.5 * N * 32.0 * job.ds) + 1e-9));
   }
+  pl.precision = precision;
+  pl.vox_kernel = vox_kernel;
+  pl.layout = layout;
+  pl.delta = delta;
+  return XRP_OK;
+}
+
+int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
+                void* out_dev, xrp_render_stats* st) {
+  using namespace xrp;
+  Scene& s = *sc->s;
+  const int nv = (int)job.views.size();
+  const int res = job.res;
+  HIPCHK(hipSetDevice(ctx->device));
+  LaunchPlan pl;
+  int rc = plan_job(ctx, sc, job, pl);
+  if (rc) return rc;
+  const int precision = pl.precision;
+  const bool vox_kernel = pl.vox_kernel;
+  const int layout = pl.layout;
+  const double delta = pl.delta;
+
+  // Sample table (device copy cached per context).
+  const double smin = job.R - kCubeHalfDiagonal, smax = job.R + kCubeHalfDiagonal;
+  if (ctx->table_mode != job.integrator || ctx->table_smin != smin ||
+      ctx->table_smax != smax || ctx->table_ds != job.ds) {
+    ctx->table_host = pl.table;
+    HIPCHK(ctx->table.ensure(std::max<size_t>(1, ctx->table_host.size()) *
+                             sizeof(double)));
+    if (!ctx->table_host.empty())
+      HIPCHK(hipMemcpy(ctx->table.p, ctx->table_host.data(),
+                       ctx->table_host.size() * sizeof(double),
+                       hipMemcpyHostToDevice));
+    ctx->table_mode = job.integrator;
+    ctx->table_smin = smin;
+    ctx->table_smax = smax;
+    ctx->table_ds = job.ds;
+  }
+
+  HIPCHK(ctx->nodes.ensure(pl.nodes.size() * sizeof(DevNode)));
+  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, s.ddefs.size()) * sizeof(DevDeform)));
+  HIPCHK(ctx->views.ensure(nv * sizeof(DevView)));
+  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
+  HIPCHK(hipMemcpyAsync(ctx->nodes.p, pl.nodes.data(),
+                        pl.nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice,
+                        ctx->stream));
+  if (!s.ddefs.empty())
+    HIPCHK(hipMemcpyAsync(ctx->defs.p, s.ddefs.data(),
+                          s.ddefs.size() * sizeof(DevDeform),
+                          hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->views.p, job.views.data(), nv * sizeof(DevView),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long),
+                        ctx->stream));
+
+  MarchParams mp = pl.mp;
+  mp.nodes = (const DevNode*)ctx->nodes.p;
+  mp.defs = (const DevDeform*)ctx->defs.p;
+  mp.views = (const DevView*)ctx->views.p;
+  mp.s_table = (const double*)ctx->table.p;
+  mp.stats = (unsigned long long*)ctx->stats.p;
+  (void)res;
 
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
   const int kMaxGridY = 65535;
@@ -809,6 +850,429 @@ int xrp_render_volume_projections(xrp_context* ctx, const float* volume, int nx,
   }
   xrp_scene_destroy(sc);
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched rendering.
+// ---------------------------------------------------------------------------
+struct xrp_batch {
+  xrp_context* ctx = nullptr;
+  int n_items = 0;
+  std::vector<int> res, n_views;
+  std::vector<xrp::MarchParams> mps;  // per item, device pointers filled in
+  std::vector<int> prec, integ;
+  std::vector<double> cost;           // per item: estimated work per tile
+  struct Group {
+    int prec = 0, integ = 0;
+    std::vector<int> items;           // ordered by cost, most expensive first
+    int first = 0;                    // offset of the group's jobs in `jobs`
+    uint32_t n_tiles = 0;
+  };
+  std::vector<Group> groups;
+  DevBuf nodes, defs, views, tables, jobs, tile_start, counters, t_first, t_done, stats, out;
+  std::vector<size_t> out_off;        // floats
+  xrp_tile_shape shape{1, 1, 2, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int n_cu = 0;
+  double ticks_per_ms = 0.0;
+  std::vector<double> latency;
+  std::string signature;
+  ~xrp_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (DevBuf* b : {&nodes, &defs, &views, &tables, &jobs, &tile_start, &counters, &t_first,
+                      &t_done, &stats, &out})
+      b->release();
+  }
+};
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, xrp_tile_shape> g_tune_table;
+
+bool valid_shape(const xrp_tile_shape& t) {
+  return t.wave_rows >= 1 && t.wave_cols >= 1 && t.wave_rows * t.wave_cols <= 4 &&
+         t.groups_per_cu >= 1 && t.groups_per_cu <= 16;
+}
+
+bool lookup_tuned(const std::string& sig, xrp_tile_shape& out) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune_table.find(sig);
+  if (it != g_tune_table.end()) {
+    out = it->second;
+    return true;
+  }
+  const char* path = std::getenv("XRP_TUNE_CACHE");
+  if (!path || !*path) return false;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  char key[4096];
+  xrp_tile_shape t{0, 0, 0, 0};
+  bool found = false;
+  while (std::fscanf(f, "%4095s %d %d %d", key, &t.wave_rows, &t.wave_cols,
+                     &t.groups_per_cu) == 4) {
+    if (sig == key && valid_shape(t)) {
+      out = t;  // the last record wins
+      found = true;
+    }
+  }
+  std::fclose(f);
+  if (found) g_tune_table[sig] = out;
+  return found;
+}
+
+void record_tuned(const std::string& sig, const xrp_tile_shape& t) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune_table[sig] = t;
+  const char* path = std::getenv("XRP_TUNE_CACHE");
+  if (!path || !*path || sig.size() >= 4096) return;
+  if (FILE* f = std::fopen(path, "a")) {
+    std::fprintf(f, "%s %d %d %d\n", sig.c_str(), t.wave_rows, t.wave_cols, t.groups_per_cu);
+    std::fclose(f);
+  }
+}
+
+// Tile the queue for b->shape and capture reset + persistent launches.
+int apply_shape(xrp_batch* b) {
+  using namespace xrp;
+  xrp_context* ctx = b->ctx;
+  const int ti = 8 * b->shape.wave_rows, tj = 8 * b->shape.wave_cols;
+  std::vector<uint32_t> starts;
+  std::vector<size_t> group_start_off;
+  for (auto& g : b->groups) {
+    group_start_off.push_back(starts.size());
+    uint64_t acc = 0;
+    for (int it : g.items) {
+      starts.push_back((uint32_t)acc);
+      const uint64_t tiles = (uint64_t)((b->res[it] + ti - 1) / ti) *
+                             (uint64_t)((b->res[it] + tj - 1) / tj) * (uint64_t)b->n_views[it];
+      acc += tiles;
+    }
+    starts.push_back((uint32_t)acc);
+    if (acc >= (1ull << 31)) return fail(XRP_ERR_DIMS, "batch holds more than 2^31 tiles");
+    g.n_tiles = (uint32_t)acc;
+  }
+  HIPCHK(b->tile_start.ensure(starts.size() * sizeof(uint32_t)));
+  HIPCHK(hipMemcpy(b->tile_start.p, starts.data(), starts.size() * sizeof(uint32_t),
+                   hipMemcpyHostToDevice));
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int e = launch_batch_reset((uint32_t*)b->counters.p, (int)b->groups.size(),
+                             (unsigned long long*)b->t_first.p,
+                             (unsigned long long*)b->t_done.p, b->n_items,
+                             (unsigned long long*)b->stats.p, b->n_items * ST_COUNT,
+                             ctx->stream);
+  for (size_t gi = 0; gi < b->groups.size() && e == 0; ++gi) {
+    const auto& g = b->groups[gi];
+    BatchGroup bg;
+    std::memset(&bg, 0, sizeof bg);
+    bg.jobs = (const MarchParams*)b->jobs.p + g.first;
+    bg.tile_start = (const uint32_t*)b->tile_start.p + group_start_off[gi];
+    bg.counter = (uint32_t*)b->counters.p + gi;
+    bg.t_first = (unsigned long long*)b->t_first.p + g.first;
+    bg.t_done = (unsigned long long*)b->t_done.p + g.first;
+    bg.n_jobs = (int)g.items.size();
+    bg.n_tiles = g.n_tiles;
+    bg.wave_rows = b->shape.wave_rows;
+    bg.wave_cols = b->shape.wave_cols;
+    const int wgs = std::max(1, b->n_cu * b->shape.groups_per_cu);
+    e = launch_batch(bg, g.prec, g.integ, wgs, ctx->stream);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+  if (e != 0) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(XRP_ERR_DEVICE, std::string("batch launch capture failed: ") +
+                                    hipGetErrorString((hipError_t)e));
+  }
+  if (ce != hipSuccess)
+    return fail(XRP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+  b->graph = graph;
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return XRP_OK;
+}
+
+int batch_run_locked(xrp_batch* b, xrp_batch_stats* st) {
+  using namespace xrp;
+  xrp_context* ctx = b->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  std::vector<unsigned long long> hst((size_t)b->n_items * ST_COUNT), t0(b->n_items),
+      t1(b->n_items);
+  HIPCHK(hipMemcpyAsync(hst.data(), b->stats.p, hst.size() * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(t0.data(), b->t_first.p, t0.size() * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(t1.data(), b->t_done.p, t1.size() * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  // t_first / t_done are stored in job (group) order
+  unsigned long long start = ~0ull;
+  for (auto v : t0) start = std::min(start, v);
+  b->latency.assign(b->n_items, 0.0);
+  for (auto& g : b->groups)
+    for (size_t k = 0; k < g.items.size(); ++k) {
+      const unsigned long long done = t1[g.first + k];
+      b->latency[g.items[k]] =
+          done >= start && b->ticks_per_ms > 0 ? (double)(done - start) / b->ticks_per_ms : 0.0;
+    }
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    st->kernel_ms = ms;
+    for (int it = 0; it < b->n_items; ++it) {
+      st->rays += (uint64_t)b->n_views[it] * b->res[it] * b->res[it];
+      st->samples += hst[(size_t)it * ST_COUNT + ST_SAMPLES];
+      st->reevaluated += hst[(size_t)it * ST_COUNT + ST_REEVAL];
+    }
+    for (auto& g : b->groups) st->tiles += g.n_tiles;
+    st->launches = (int)b->groups.size();
+  }
+  return XRP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xrp_batch_create(xrp_context* ctx, const xrp_batch_item* items, int n_items,
+                     xrp_batch** out) {
+  using namespace xrp;
+  if (!ctx || !items || !out) return fail(XRP_ERR_NULL, "NULL argument");
+  *out = nullptr;
+  if (n_items <= 0) return fail(XRP_ERR_DIMS, "n_items must be positive");
+  for (int it = 0; it < n_items; ++it) {
+    if (!items[it].scene || !items[it].views)
+      return fail(XRP_ERR_NULL, "item " + std::to_string(it) + ": NULL scene or views");
+    int rc = check_params(&items[it].params, items[it].n_views);
+    if (rc) return fail(rc, "item " + std::to_string(it) + ": " + g_error);
+    if (items[it].params.output_format != XRP_OUTPUT_F32)
+      return fail(XRP_ERR_ARG, "item " + std::to_string(it) + ": batches write float32 images");
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  std::unique_ptr<xrp_batch> b(new xrp_batch());
+  b->ctx = ctx;
+  b->n_items = n_items;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+  b->n_cu = prop.multiProcessorCount;
+  int khz = 0;
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  b->ticks_per_ms = (double)khz;
+
+  // Plan every item; concatenate nodes, deformations, views and tables.
+  std::vector<DevNode> all_nodes;
+  std::vector<DevDeform> all_defs;
+  std::vector<DevView> all_views;
+  std::vector<double> all_tables;
+  std::vector<size_t> node_off, def_off, view_off, table_off;
+  size_t out_floats = 0;
+  std::string sig = std::string(prop.gcnArchName) + "/" + std::to_string(b->n_cu);
+  for (int it = 0; it < n_items; ++it) {
+    xrp_scene* sc = const_cast<xrp_scene*>(items[it].scene);
+    RenderJob job = make_job(&items[it].params, items[it].views, items[it].n_views);
+    LaunchPlan pl;
+    int rc = plan_job(ctx, sc, job, pl, /*allow_voxel_kernel=*/false);
+    if (rc) return rc;
+    const Scene& s = *sc->s;
+    node_off.push_back(all_nodes.size());
+    def_off.push_back(all_defs.size());
+    view_off.push_back(all_views.size());
+    table_off.push_back(all_tables.size());
+    all_nodes.insert(all_nodes.end(), pl.nodes.begin(), pl.nodes.end());
+    all_defs.insert(all_defs.end(), s.ddefs.begin(), s.ddefs.end());
+    all_views.insert(all_views.end(), job.views.begin(), job.views.end());
+    all_tables.insert(all_tables.end(), pl.table.begin(), pl.table.end());
+    b->res.push_back(job.res);
+    b->n_views.push_back((int)job.views.size());
+    b->out_off.push_back(out_floats);
+    out_floats += (size_t)job.views.size() * job.res * job.res;
+    b->mps.push_back(pl.mp);
+    b->prec.push_back(pl.precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED);
+    b->integ.push_back(job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
+                                                                   : INT_SIMPLE);
+    // work per tile ~ nodes x samples in the clip box (x2 for FP64)
+    double frac = 1.0;
+    if (pl.mp.clip) {
+      double ext = 0;
+      for (int k = 0; k < 3; ++k) ext += (pl.mp.hi[k] - pl.mp.lo[k]) * (pl.mp.hi[k] - pl.mp.lo[k]);
+      frac = std::min(1.0, std::sqrt(ext) / (2 * kCubeHalfDiagonal));
+    }
+    b->cost.push_back((double)pl.nodes.size() * pl.mp.K * frac *
+                      (b->prec.back() == PREC_FP64 ? 2.0 : 1.0));
+    sig += ";" + std::to_string(pl.nodes.size()) + "," + std::to_string(job.res) + "," +
+           std::to_string(job.views.size()) + "," + std::to_string(pl.mp.K) + "," +
+           std::to_string(b->prec.back()) + "," + std::to_string(b->integ.back());
+  }
+  b->signature = sig;
+
+  // Groups by (precision, integrator), items by cost, most expensive first.
+  for (int it = 0; it < n_items; ++it) {
+    auto g = std::find_if(b->groups.begin(), b->groups.end(), [&](const xrp_batch::Group& x) {
+      return x.prec == b->prec[it] && x.integ == b->integ[it];
+    });
+    if (g == b->groups.end()) {
+      b->groups.push_back(xrp_batch::Group());
+      g = b->groups.end() - 1;
+      g->prec = b->prec[it];
+      g->integ = b->integ[it];
+    }
+    g->items.push_back(it);
+  }
+  int first = 0;
+  for (auto& g : b->groups) {
+    std::stable_sort(g.items.begin(), g.items.end(),
+                     [&](int a, int c) { return b->cost[a] > b->cost[c]; });
+    g.first = first;
+    first += (int)g.items.size();
+  }
+
+  // Device copies.
+  HIPCHK(b->nodes.ensure(std::max<size_t>(1, all_nodes.size()) * sizeof(DevNode)));
+  HIPCHK(b->defs.ensure(std::max<size_t>(1, all_defs.size()) * sizeof(DevDeform)));
+  HIPCHK(b->views.ensure(all_views.size() * sizeof(DevView)));
+  HIPCHK(b->tables.ensure(std::max<size_t>(1, all_tables.size()) * sizeof(double)));
+  HIPCHK(b->jobs.ensure((size_t)n_items * sizeof(MarchParams)));
+  HIPCHK(b->counters.ensure(b->groups.size() * sizeof(uint32_t)));
+  HIPCHK(b->t_first.ensure((size_t)n_items * sizeof(unsigned long long)));
+  HIPCHK(b->t_done.ensure((size_t)n_items * sizeof(unsigned long long)));
+  HIPCHK(b->stats.ensure((size_t)n_items * ST_COUNT * sizeof(unsigned long long)));
+  HIPCHK(b->out.ensure(out_floats * sizeof(float)));
+  if (!all_nodes.empty())
+    HIPCHK(hipMemcpy(b->nodes.p, all_nodes.data(), all_nodes.size() * sizeof(DevNode),
+                     hipMemcpyHostToDevice));
+  if (!all_defs.empty())
+    HIPCHK(hipMemcpy(b->defs.p, all_defs.data(), all_defs.size() * sizeof(DevDeform),
+                     hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->views.p, all_views.data(), all_views.size() * sizeof(DevView),
+                   hipMemcpyHostToDevice));
+  if (!all_tables.empty())
+    HIPCHK(hipMemcpy(b->tables.p, all_tables.data(), all_tables.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+  std::vector<MarchParams> jobs(n_items);
+  for (auto& g : b->groups)
+    for (size_t k = 0; k < g.items.size(); ++k) {
+      const int it = g.items[k];
+      MarchParams mp = b->mps[it];
+      mp.nodes = (const DevNode*)b->nodes.p + node_off[it];
+      mp.defs = (const DevDeform*)b->defs.p + def_off[it];
+      mp.views = (const DevView*)b->views.p + view_off[it];
+      mp.s_table = (const double*)b->tables.p + table_off[it];
+      mp.stats = (unsigned long long*)b->stats.p + (size_t)it * ST_COUNT;
+      mp.out = (float*)b->out.p + b->out_off[it];
+      mp.view_offset = 0;
+      mp.n_views = b->n_views[it];
+      b->mps[it] = mp;
+      jobs[g.first + k] = mp;
+    }
+  HIPCHK(hipMemcpy(b->jobs.p, jobs.data(), jobs.size() * sizeof(MarchParams),
+                   hipMemcpyHostToDevice));
+
+  xrp_tile_shape tuned;
+  if (lookup_tuned(b->signature, tuned)) b->shape = tuned;
+  int rc = apply_shape(b.get());
+  if (rc) return rc;
+  *out = b.release();
+  return XRP_OK;
+}
+
+int xrp_batch_set_shape(xrp_batch* b, const xrp_tile_shape* shape) {
+  if (!b || !shape) return fail(XRP_ERR_NULL, "NULL argument");
+  if (!valid_shape(*shape))
+    return fail(XRP_ERR_ARG, "tile shape: 1 <= rows, cols; rows*cols <= 4; "
+                             "1 <= groups_per_cu <= 16");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  HIPCHK(hipSetDevice(b->ctx->device));
+  b->shape = *shape;
+  b->shape.reserved = 0;
+  return apply_shape(b);
+}
+
+int xrp_batch_run(xrp_batch* b, xrp_batch_stats* st) {
+  if (!b) return fail(XRP_ERR_NULL, "NULL batch");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  return batch_run_locked(b, st);
+}
+
+int xrp_batch_tune(xrp_batch* b, int repeats, xrp_tile_shape* chosen) {
+  if (!b) return fail(XRP_ERR_NULL, "NULL batch");
+  if (repeats < 1) repeats = 1;
+  static const xrp_tile_shape cands[] = {
+      {1, 1, 2, 0}, {1, 1, 4, 0}, {1, 2, 2, 0}, {2, 1, 2, 0}, {2, 2, 1, 0},
+      {2, 2, 2, 0}, {2, 2, 4, 0}, {1, 4, 2, 0}, {4, 1, 2, 0}};
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  HIPCHK(hipSetDevice(b->ctx->device));
+  double best_ms = INFINITY;
+  xrp_tile_shape best = b->shape;
+  for (const auto& c : cands) {
+    b->shape = c;
+    int rc = apply_shape(b);
+    if (rc) return rc;
+    double ms = INFINITY;
+    for (int r = 0; r < repeats; ++r) {
+      xrp_batch_stats st;
+      rc = batch_run_locked(b, &st);
+      if (rc) return rc;
+      ms = std::min(ms, st.kernel_ms);
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  b->shape = best;
+  int rc = apply_shape(b);
+  if (rc) return rc;
+  record_tuned(b->signature, best);
+  if (chosen) *chosen = best;
+  return XRP_OK;
+}
+
+int xrp_batch_output(const xrp_batch* b, int item, const float** device_images) {
+  if (!b || !device_images) return fail(XRP_ERR_NULL, "NULL argument");
+  if (item < 0 || item >= b->n_items) return fail(XRP_ERR_ARG, "item out of range");
+  *device_images = (const float*)b->out.p + b->out_off[item];
+  return XRP_OK;
+}
+
+int xrp_batch_read(xrp_batch* b, int item, float* host_images) {
+  if (!b || !host_images) return fail(XRP_ERR_NULL, "NULL argument");
+  if (item < 0 || item >= b->n_items) return fail(XRP_ERR_ARG, "item out of range");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  HIPCHK(hipSetDevice(b->ctx->device));
+  const size_t n = (size_t)b->n_views[item] * b->res[item] * b->res[item];
+  HIPCHK(hipMemcpy(host_images, (const float*)b->out.p + b->out_off[item], n * sizeof(float),
+                   hipMemcpyDeviceToHost));
+  return XRP_OK;
+}
+
+int xrp_batch_latencies(const xrp_batch* b, double* ms) {
+  if (!b || !ms) return fail(XRP_ERR_NULL, "NULL argument");
+  for (int it = 0; it < b->n_items; ++it)
+    ms[it] = it < (int)b->latency.size() ? b->latency[it] : 0.0;
+  return XRP_OK;
+}
+
+void xrp_batch_destroy(xrp_batch* b) {
+  if (!b) return;
+  {
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+  }
+  delete b;
 }
 
 }  // extern "C"

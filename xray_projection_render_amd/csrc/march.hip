@@ -22,6 +22,7 @@
 
 #include "eval_device.hpp"
 #include "march.hpp"
+#include "march_ray.hpp"
 
 namespace xrp {
 
@@ -33,159 +34,9 @@ __global__ __launch_bounds__(256) void march_kernel(MarchParams p) {
   const int ti = blockIdx.x / tiles_j, tj = blockIdx.x - ti * tiles_j;
   const int i = ti * 16 + (wave >> 1) * 8 + (lane >> 3);
   const int j = tj * 16 + (wave & 1) * 8 + (lane & 7);
-  const int view = p.view_offset + blockIdx.y;
-  const bool valid = i < p.res && j < p.res;
-  const DevView& V = p.views[view];
-
-  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0;
-  double T = p.flat_field;
-
-  if (valid) {
-    Ray r = make_ray(p, V, i, j);
-    double t_in, t_out;
-    if (clip_ray(p, r, t_in, t_out)) {
-      if constexpr (INTEG == INT_SIMPLE) {
-        // main.go:144-154; s_k = s_table[k], s_{k+1} = s_k + ds.
-        long k_lo = 0, k_hi = p.K - 1;
-        if (p.clip) {
-          double a = floor((t_in - p.smin) / p.ds) - 2.0;
-          double b = ceil((t_out - p.smin) / p.ds) + 2.0;
-          k_lo = a > 0.0 ? (long)a : 0;
-          k_hi = b < (double)(p.K - 1) ? (long)b : (long)(p.K - 1);
-        }
-        const double ds = p.ds;
-        if constexpr (PREC == PREC_FP64) {
-          if (k_lo <= k_hi) {
-            double s = p.s_table[k_lo];
-            for (long k = k_lo; k <= k_hi; ++k) {
-              double x = r.o[0] + r.d[0] * s;
-              double y = r.o[1] + r.d[1] * s;
-              double z = r.o[2] + r.d[2] * s;
-              T += (density64(p, x, y, z) * p.dm) * ds;
-              s += ds;
-            }
-            n_samples += (unsigned long long)(k_hi - k_lo + 1);
-          }
-        } else {
-          const float ds32 = (float)ds;
-          const float d0 = (float)r.d[0], d1 = (float)r.d[1], d2 = (float)r.d[2];
-          for (long k0 = k_lo; k0 <= k_hi; k0 += kChunk) {
-            double sa = p.s_table[k0];
-            float a0 = (float)(r.o[0] + r.d[0] * sa);
-            float a1 = (float)(r.o[1] + r.d[1] * sa);
-            float a2 = (float)(r.o[2] + r.d[2] * sa);
-            int n = (int)min((long)kChunk, k_hi - k0 + 1);
-            for (int jj = 0; jj < n; ++jj) {
-              float sj = (float)jj * ds32;
-              float x = fmaf(d0, sj, a0), y = fmaf(d1, sj, a1),
-                    z = fmaf(d2, sj, a2);
-              deform32(p.defs, p.n_defs, x, y, z);
-              GR g = root32(p.nodes, x, y, z);
-              double rho = g.v;
-              if (g.amb || p.audit) {
-                double s = p.s_table[k0 + jj];
-                double e = density64(p, r.o[0] + r.d[0] * s,
-                                     r.o[1] + r.d[1] * s, r.o[2] + r.d[2] * s);
-                if (g.amb) {
-                  ++n_reeval;
-                } else if (e != rho) {
-                  ++n_miss;
-                }
-                rho = e;
-              }
-              T += (rho * p.dm) * ds;
-            }
-            n_samples += (unsigned long long)n;
-          }
-        }
-      } else {
-        // main.go:159-199: coarse right-endpoint samples R[n] (s_table),
-        // fine sub-steps of DS/10 from the previous coarse point where
-        // (rho == 0) toggles.
-        const double DS = p.ds, dsf = p.ds_fine;
-        long n_lo = 0, n_hi = p.K - 1;
-        if (p.clip) {
-          double a = floor((t_in - p.smin) / DS) - 3.0;
-          double b = ceil((t_out - p.smin) / DS) + 3.0;
-          n_lo = a > 0.0 ? (long)a : 0;
-          n_hi = b < (double)(p.K - 1) ? (long)b : (long)(p.K - 1);
-        }
-        bool prev_zero = true;
-        for (long n = n_lo; n <= n_hi; ++n) {
-          double right = p.s_table[n];
-          double rho;
-          {
-            double x = r.o[0] + r.d[0] * right;
-            double y = r.o[1] + r.d[1] * right;
-            double z = r.o[2] + r.d[2] * right;
-            if constexpr (PREC == PREC_FP64) {
-              rho = density64(p, x, y, z) * p.dm;
-            } else {
-              float fx = (float)x, fy = (float)y, fz = (float)z;
-              deform32(p.defs, p.n_defs, fx, fy, fz);
-              GR g = root32(p.nodes, fx, fy, fz);
-              rho = g.v;
-              if (g.amb || p.audit) {
-                double e = density64(p, x, y, z);
-                if (g.amb)
-                  ++n_reeval;
-                else if (e != rho)
-                  ++n_miss;
-                rho = e;
-              }
-              rho = rho * p.dm;
-            }
-            ++n_samples;
-          }
-          if ((rho == 0) != prev_zero) {
-            double left = n == 0 ? p.smin : p.s_table[n - 1];
-            left += dsf;
-            while (left < right) {
-              double x = r.o[0] + r.d[0] * left;
-              double y = r.o[1] + r.d[1] * left;
-              double z = r.o[2] + r.d[2] * left;
-              double rf;
-              if constexpr (PREC == PREC_FP64) {
-                rf = density64(p, x, y, z);
-              } else {
-                float fx = (float)x, fy = (float)y, fz = (float)z;
-                deform32(p.defs, p.n_defs, fx, fy, fz);
-                GR g = root32(p.nodes, fx, fy, fz);
-                rf = g.v;
-                if (g.amb || p.audit) {
-                  double e = density64(p, x, y, z);
-                  if (g.amb)
-                    ++n_reeval;
-                  else if (e != rf)
-                    ++n_miss;
-                  rf = e;
-                }
-              }
-              T += (rf * p.dm) * dsf;
-              left += dsf;
-              ++n_fine;
-            }
-            T += rho * dsf;
-          } else {
-            T += rho * DS;
-          }
-          prev_zero = rho == 0;
-        }
-      }
-    }
-    store_pixel(p, blockIdx.y, i, j, T);
-  }
-
-  if (p.stats) {
-    unsigned long long a = wave_sum(n_samples), b = wave_sum(n_reeval),
-                       c = wave_sum(n_miss), d = wave_sum(n_fine);
-    if (lane == 0) {
-      if (a) atomicAdd(p.stats + ST_SAMPLES, a);
-      if (b) atomicAdd(p.stats + ST_REEVAL, b);
-      if (c) atomicAdd(p.stats + ST_AUDIT_MISS, c);
-      if (d) atomicAdd(p.stats + ST_FINE, d);
-    }
-  }
+  RayCounts rc;
+  if (i < p.res && j < p.res) march_ray<PREC, INTEG>(p, blockIdx.y, i, j, rc);
+  flush_counts(p.stats, rc);
 }
 
 // Volume export (main.go:549-635): density() at x = i/res*2-1 (the i/N

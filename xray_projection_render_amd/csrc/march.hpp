@@ -84,6 +84,28 @@ inline bool voxel_layout_linear(int layout) {
 }
 int launch_voxel(const MarchParams& p, int layout, void* stream);
 
+// Batched persistent launch: every (scene, view, tile) item of one
+// (precision, integrator) group is taken from an atomic work counter by the
+// waves of a persistent grid until the counter passes n_tiles.
+struct BatchGroup {
+  const MarchParams* jobs;      // one per scene of the group, ordered by cost
+  const uint32_t* tile_start;   // n_jobs + 1 prefix sums of tiles per job
+  uint32_t* counter;            // work counter, zeroed by launch_batch_reset
+  unsigned long long* t_first;  // per job: wall clock at its first tile start
+  unsigned long long* t_done;   // per job: wall clock at its last tile end
+  int32_t n_jobs;
+  uint32_t n_tiles;
+  int32_t wave_rows, wave_cols; // item: (8 wave_rows) x (8 wave_cols) pixels, one wave
+  int32_t pad_;
+};
+int launch_batch(const BatchGroup& g, int precision, int integrator, int workgroups,
+                 void* stream);
+// Zero the work counters and per-job stats, and reset the timestamps,
+// with a kernel (no memset nodes, so the sequence can be graph-captured).
+int launch_batch_reset(uint32_t* counters, int n_counters, unsigned long long* t_first,
+                       unsigned long long* t_done, int n_jobs, unsigned long long* stats,
+                       int n_stats, void* stream);
+
 // Volume export: densities (FP64) into dens, max into *maxbits (must start
 // at 0), bytes into out; res^3 entries each.
 int launch_volume(const MarchParams& p, int res, double* dens,
