@@ -1,0 +1,277 @@
+// eval_block.hpp - the guarded FP32 evaluator of eval_device.hpp applied to
+// a block of S consecutive samples of one ray per node visit.
+//
+// Every lane of a wave visits the same node at the same time, so the node's
+// record is read (scalar loads) and its kind branched on once per block
+// instead of once per sample, and the S independent membership tests give
+// the wave instruction-level parallelism to cover the load latency.  The
+// FP32 arithmetic per sample is the one of the single-sample functions
+// (sphere32, cyl32, ... in eval_device.hpp), so membership, guard flags and
+// densities are identical; a collection stops walking its children for a
+// sample exactly where the single-sample walk would (first ambiguous child,
+// or first positive child of a greedy collection).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "eval_device.hpp"
+
+namespace xrp {
+
+template <int S>
+struct PBlock {
+  float x[S], y[S], z[S];
+};
+
+template <int S>
+__device__ __forceinline__ void sphere_blk(const DevNode& n, const PBlock<S>& p,
+                                           double (&v)[S], unsigned& amb) {
+  const float cx = n.f[0], cy = n.f[1], cz = n.f[2], r2 = n.f[3], g = n.g[0];
+  const double rho = n.d[4];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    float dx = p.x[s] - cx, dy = p.y[s] - cy, dz = p.z[s] - cz;
+    float q = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, -r2)));
+    v[s] = q < 0.0f ? rho : 0.0;
+    amb |= (fabsf(q) < g ? 1u : 0u) << s;
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void box_blk(const DevNode& n, const PBlock<S>& p,
+                                        double (&v)[S], unsigned& amb) {
+  const float cx = n.f[0], cy = n.f[1], cz = n.f[2], hx = n.f[3], hy = n.f[4], hz = n.f[5];
+  const float g0 = n.g[0], g1 = n.g[1], g2 = n.g[2];
+  const double rho = n.d[6];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    float ax = fabsf(p.x[s] - cx) - hx;
+    float ay = fabsf(p.y[s] - cy) - hy;
+    float az = fabsf(p.z[s] - cz) - hz;
+    bool out = ax >= g0 || ay >= g1 || az >= g2;
+    bool in = ax < -g0 && ay < -g1 && az < -g2;
+    v[s] = in ? rho : 0.0;
+    amb |= (!(in || out) ? 1u : 0u) << s;
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void pped_blk(const DevNode& n, const PBlock<S>& p,
+                                         double (&v)[S], unsigned& amb) {
+  const float ox = n.f[0], oy = n.f[1], oz = n.f[2];
+  const float m0 = n.f[3], m1 = n.f[4], m2 = n.f[5], m3 = n.f[6], m4 = n.f[7], m5 = n.f[8],
+              m6 = n.f[9], m7 = n.f[10], m8 = n.f[11];
+  const float g0 = n.g[0], g1 = n.g[1], g2 = n.g[2];
+  const double rho = n.d[12];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    float w0 = p.x[s] - ox, w1 = p.y[s] - oy, w2 = p.z[s] - oz;
+    float u = fmaf(m0, w0, fmaf(m3, w1, m6 * w2));
+    float vv = fmaf(m1, w0, fmaf(m4, w1, m7 * w2));
+    float t = fmaf(m2, w0, fmaf(m5, w1, m8 * w2));
+    bool out = u <= -g0 || u >= 1.0f + g0 || vv <= -g1 || vv >= 1.0f + g1 || t <= -g2 ||
+               t >= 1.0f + g2;
+    bool in = u > g0 && u < 1.0f - g0 && vv > g1 && vv < 1.0f - g1 && t > g2 &&
+              t < 1.0f - g2;
+    v[s] = in ? rho : 0.0;
+    amb |= (!(in || out) ? 1u : 0u) << s;
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
+                                        double (&v)[S], unsigned& amb) {
+  const float px = n.f[0], py = n.f[1], pz = n.f[2];
+  const float v0 = n.f[3], v1 = n.f[4], v2 = n.f[5], vv = n.f[6], ivv = n.f[7], r2 = n.f[8];
+  const float gq = n.g[0], gt = n.g[1];
+  const double rho = n.d[8];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    float w0 = p.x[s] - px, w1 = p.y[s] - py, w2 = p.z[s] - pz;
+    float t = fmaf(w0, v0, fmaf(w1, v1, w2 * v2));
+    float c = t * ivv;
+    float e0 = fmaf(-c, v0, w0), e1 = fmaf(-c, v1, w1), e2 = fmaf(-c, v2, w2);
+    float q = fmaf(e0, e0, fmaf(e1, e1, fmaf(e2, e2, -r2)));
+    bool cap_out = t < -gt || t > vv + gt;
+    bool cap_in = t > gt && t < vv - gt;
+    bool in = cap_in && q < -gq;
+    bool out = cap_out || q >= gq;
+    v[s] = in ? rho : 0.0;
+    amb |= (!(in || out) ? 1u : 0u) << s;
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
+                                           double (&v)[S], unsigned& amb) {
+  const float cx = n.f[0], cy = n.f[1], cz = n.f[2], sc = n.f[3], th = n.f[4], g = n.g[0];
+  const double rho = n.d[5];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    float qx = (p.x[s] - cx) * sc, qy = (p.y[s] - cy) * sc, qz = (p.z[s] - cz) * sc;
+    float sx, cxs, sy, cys, sz, czs;
+    sincosf(qx, &sx, &cxs);
+    sincosf(qy, &sy, &cys);
+    sincosf(qz, &sz, &czs);
+    float gg = fmaf(sx, cys, fmaf(sy, czs, sz * cxs));
+    float a = fabsf(gg) - th;
+    v[s] = a < 0.0f ? rho : 0.0;
+    amb |= (fabsf(a) < g ? 1u : 0u) << s;
+  }
+}
+
+template <int D, int S>
+__device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
+                                         const PBlock<S>& p, unsigned live, double (&out)[S],
+                                         unsigned& amb_out);
+
+// Tessellation wrap (objects.go:568-582) per sample, then the unit cell's
+// collection for the samples that passed the bounds and seam guards.
+template <int D, int S>
+__device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
+                                         const PBlock<S>& p, unsigned live, double (&out)[S],
+                                         unsigned& amb_out) {
+  const DevNode& n = N[i];
+  const float b0 = n.f[0], b1 = n.f[1], b2 = n.f[2], b3 = n.f[3], b4 = n.f[4], b5 = n.f[5];
+  const float gb = n.g[1], gw = n.g[0];
+  PBlock<S> q;
+  unsigned go = 0, amb = 0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const float x = p.x[s], y = p.y[s], z = p.z[s];
+    out[s] = 0.0;
+    q.x[s] = x;
+    q.y[s] = y;
+    q.z[s] = z;
+    bool o = x < b0 - gb || x > b1 + gb || y < b2 - gb || y > b3 + gb || z < b4 - gb ||
+             z > b5 + gb;
+    bool in = x > b0 + gb && x < b1 - gb && y > b2 + gb && y < b3 - gb && z > b4 + gb &&
+              z < b5 - gb;
+    float fx = (x - n.f[6]) * n.f[12];
+    float fy = (y - n.f[7]) * n.f[13];
+    float fz = (z - n.f[8]) * n.f[14];
+    float nx = floorf(fx), ny = floorf(fy), nz = floorf(fz);
+    float rx = fx - nx, ry = fy - ny, rz = fz - nz;
+    bool seam = rx < gw || rx > 1.0f - gw || ry < gw || ry > 1.0f - gw || rz < gw ||
+                rz > 1.0f - gw;
+    q.x[s] = fmaf(-n.f[9], nx, x);
+    q.y[s] = fmaf(-n.f[10], ny, y);
+    q.z[s] = fmaf(-n.f[11], nz, z);
+    if (!o) {
+      if (!in || seam)
+        amb |= 1u << s;
+      else
+        go |= 1u << s;
+    }
+  }
+  amb &= live;
+  go &= live;
+  amb_out |= amb;
+  if constexpr (D + 1 <= kMaxDepth) {
+    if (go) coll_blk<D + 1, S>(N, n.first, q, go, out, amb_out);
+  } else {
+    amb_out |= go;
+  }
+}
+
+template <int D, int S>
+__device__ __forceinline__ void leaf_blk(const DevNode* __restrict__ N, int c,
+                                         const PBlock<S>& p, unsigned live, double (&v)[S],
+                                         unsigned& amb) {
+  const DevNode& k = N[c];
+  switch (k.kind) {
+    case K_SPHERE: sphere_blk<S>(k, p, v, amb); return;
+    case K_BOX: box_blk<S>(k, p, v, amb); return;
+    case K_PPED: pped_blk<S>(k, p, v, amb); return;
+    case K_CYL: cyl_blk<S>(k, p, v, amb); return;
+    case K_GYROID: gyroid_blk<S>(k, p, v, amb); return;
+    case K_TESS:
+      // Tessellations below the first level (a tessellation inside a
+      // tessellated cell) are left to the FP64 evaluator: the blocked
+      // walk keeps one level of cell state in registers.
+      if constexpr (D == 0) {
+        tess_blk<D, S>(N, c, p, live, v, amb);
+      } else {
+#pragma unroll
+        for (int s = 0; s < S; ++s) v[s] = 0.0;
+        amb |= live;
+      }
+      return;
+    case K_VOXEL:
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        GR r = voxel32(k, p.x[s], p.y[s], p.z[s]);
+        v[s] = r.v;
+        amb |= (r.amb ? 1u : 0u) << s;
+      }
+      return;
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) v[s] = 0.0;
+}
+
+// objects.go:422-438 per sample: sum then clamp to [0, 1]; greedy returns
+// the first positive child unclamped.
+template <int D, int S>
+__device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
+                                         const PBlock<S>& p, unsigned live, double (&out)[S],
+                                         unsigned& amb_out) {
+  const DevNode& n = N[i];
+  const bool greedy = n.greedy;
+  // out[s] holds the running sum until sample s finishes
+#pragma unroll
+  for (int s = 0; s < S; ++s) out[s] = 0.0;
+  unsigned done = 0, amb = 0, hit = 0;
+  for (int c = n.first; c < n.next; c = N[c].next) {
+    const unsigned act = live & ~done;
+    if (!act) break;
+    double v[S];
+    unsigned a = 0;
+    leaf_blk<D, S>(N, c, p, act, v, a);
+    a &= act;
+    amb |= a;
+    done |= a;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (((act & ~a) >> s) & 1u) {
+        if (greedy && v[s] > 0.0) {
+          out[s] = v[s];
+          hit |= 1u << s;
+          done |= 1u << s;
+        } else {
+          out[s] += v[s];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (((live & ~amb & ~hit) >> s) & 1u) {
+      double d = out[s];
+      out[s] = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+    }
+  }
+  amb_out |= amb;
+}
+
+template <int S>
+__device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PBlock<S>& p,
+                                         unsigned live, double (&out)[S], unsigned& amb) {
+  if (N[0].kind == K_COLL)
+    coll_blk<0, S>(N, 0, p, live, out, amb);
+  else
+    leaf_blk<0, S>(N, 0, p, live, out, amb);
+  amb &= live;
+}
+
+template <int S>
+__device__ __forceinline__ void deform_blk(const DevDeform* __restrict__ D, int nd,
+                                           PBlock<S>& p) {
+  for (int k = 0; k < nd; ++k) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) deform32(D + k, 1, p.x[s], p.y[s], p.z[s]);
+  }
+}
+
+}  // namespace xrp

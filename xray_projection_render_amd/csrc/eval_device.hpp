@@ -245,6 +245,15 @@ __device__ __forceinline__ double density64(const MarchParams& p, double x,
   return node64<0>(p.nodes, 0, x, y, z);
 }
 
+// The same as an out-of-line call, for the rare FP64 re-evaluation inside
+// the FP32 march: kept out of the caller, its registers do not add to the
+// pressure of the hot loop.
+__device__ __noinline__ double density64_call(const DevNode* nodes, const DevDeform* defs,
+                                              int n_defs, double x, double y, double z) {
+  deform64(defs, n_defs, x, y, z);
+  return node64<0>(nodes, 0, x, y, z);
+}
+
 // ---------------------------------------------------------------------------
 // Guarded FP32 evaluator.  Returns the exact FP64 density when every
 // membership decision is outside its guard band, else amb = true.

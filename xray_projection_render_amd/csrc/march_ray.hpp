@@ -14,10 +14,14 @@
 
 #include <cmath>
 
+#include "eval_block.hpp"
 #include "eval_device.hpp"
 #include "march.hpp"
 
 namespace xrp {
+
+// Samples per node visit of the guarded FP32 simple march.
+constexpr int kBlock = 4;
 
 // Counters one ray adds to the launch's stats.
 struct RayCounts {
@@ -67,25 +71,41 @@ __device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, in
             float a1 = (float)(r.o[1] + r.d[1] * sa);
             float a2 = (float)(r.o[2] + r.d[2] * sa);
             int n = (int)min((long)kChunk, k_hi - k0 + 1);
-            for (int jj = 0; jj < n; ++jj) {
-              float sj = (float)jj * ds32;
-              float x = fmaf(d0, sj, a0), y = fmaf(d1, sj, a1),
-                    z = fmaf(d2, sj, a2);
-              deform32(p.defs, p.n_defs, x, y, z);
-              GR g = root32(p.nodes, x, y, z);
-              double rho = g.v;
-              if (g.amb || p.audit) {
-                double s = p.s_table[k0 + jj];
-                double e = density64(p, r.o[0] + r.d[0] * s,
-                                     r.o[1] + r.d[1] * s, r.o[2] + r.d[2] * s);
-                if (g.amb) {
-                  ++n_reeval;
-                } else if (e != rho) {
-                  ++n_miss;
-                }
-                rho = e;
+            // kBlock consecutive samples per node visit (eval_block.hpp);
+            // T still adds the samples in order.
+            for (int jb = 0; jb < n; jb += kBlock) {
+              PBlock<kBlock> P;
+#pragma unroll
+              for (int q = 0; q < kBlock; ++q) {
+                float sj = (float)(jb + q) * ds32;
+                P.x[q] = fmaf(d0, sj, a0);
+                P.y[q] = fmaf(d1, sj, a1);
+                P.z[q] = fmaf(d2, sj, a2);
               }
-              T += (rho * p.dm) * ds;
+              deform_blk<kBlock>(p.defs, p.n_defs, P);
+              const unsigned live =
+                  n - jb >= kBlock ? (1u << kBlock) - 1u : (1u << (n - jb)) - 1u;
+              double v[kBlock];
+              unsigned amb = 0;
+              root_blk<kBlock>(p.nodes, P, live, v, amb);
+#pragma unroll
+              for (int q = 0; q < kBlock; ++q) {
+                if (!((live >> q) & 1u)) break;
+                double rho = v[q];
+                const bool a = (amb >> q) & 1u;
+                if (a || p.audit) {
+                  double s = p.s_table[k0 + jb + q];
+                  double e = density64_call(p.nodes, p.defs, p.n_defs, r.o[0] + r.d[0] * s,
+                                            r.o[1] + r.d[1] * s, r.o[2] + r.d[2] * s);
+                  if (a) {
+                    ++n_reeval;
+                  } else if (e != rho) {
+                    ++n_miss;
+                  }
+                  rho = e;
+                }
+                T += (rho * p.dm) * ds;
+              }
             }
             n_samples += (unsigned long long)n;
           }
