@@ -122,6 +122,7 @@ int xrp_abi_version(void);
 const char *xrp_last_error(void);
 
 /* ---- context -------------------------------------------------------------- */
+/* Batches and surrogates live on a context: destroy them before it. */
 int xrp_context_create(int device, xrp_context **out);
 void xrp_context_destroy(xrp_context *ctx);
 int xrp_context_synchronize(xrp_context *ctx);
@@ -256,6 +257,88 @@ int xrp_batch_read(xrp_batch *batch, int item, float *host_images);
  * last tile end in the last run (n_items values). */
 int xrp_batch_latencies(const xrp_batch *batch, double *ms);
 void xrp_batch_destroy(xrp_batch *batch);
+
+/* ---- learned density surrogate ----------------------------------------------
+ * No counterpart in the reference.  A multiresolution hash encoding (16
+ * levels x 2 features, 2^log2_table entries per level) feeding a 32-64-64-1
+ * MLP learns the scene's object-space density from exact samples; the
+ * march then evaluates the network (BF16 MFMA, FP32 accumulation) at a
+ * coarser step, and a sampled exact check re-renders any 16x16 tile whose
+ * error exceeds a tolerance with the exact march. */
+typedef struct xrp_surrogate xrp_surrogate;
+
+typedef struct {
+  uint32_t struct_size;
+  int32_t log2_table; /* entries per level: default 19 */
+  int32_t n_min;      /* coarsest level resolution: default 16 */
+  int32_t n_max;      /* finest level resolution: default 512 */
+  int32_t wrap;       /* -1: learn one unit cell when the root is a tessellation */
+  int32_t reserved;
+  uint64_t seed;      /* initialisation and sampling seed */
+} xrp_surrogate_config;
+
+typedef struct {
+  uint32_t struct_size;
+  int32_t batch;      /* samples per optimiser step: default 65536 */
+  int64_t samples;    /* exact training samples: default 2^24 */
+  int32_t epochs;     /* passes over the samples: default 4 */
+  int32_t reserved;
+  double lr_table;    /* Adam step for the tables: default 1e-2 */
+  double lr_mlp;      /* Adam step for the MLP: default 1e-3 */
+} xrp_train_config;
+
+typedef struct {
+  double first_loss;  /* mean squared error of the first step */
+  double final_loss;  /* mean over the last epoch */
+  double label_ms;    /* exact sample generation */
+  double train_ms;    /* forward/backward/Adam kernels */
+  int32_t steps;
+  int32_t reserved;
+} xrp_train_stats;
+
+typedef struct {
+  double surrogate_ms;   /* surrogate march */
+  double check_ms;       /* exact march of the check pixels */
+  double fallback_ms;    /* exact re-render of failing tiles */
+  double max_sampled_error; /* largest |surrogate - exact| at check pixels */
+  uint64_t tiles;
+  uint64_t tiles_failed;
+  uint64_t check_pixels;
+} xrp_surrogate_stats;
+
+void xrp_surrogate_default_config(xrp_surrogate_config *c);
+void xrp_train_default_config(xrp_train_config *c);
+int xrp_surrogate_create(xrp_context *ctx, const xrp_scene *scene,
+                         const xrp_surrogate_config *config, xrp_surrogate **out);
+int xrp_surrogate_train(xrp_surrogate *s, const xrp_train_config *config,
+                        xrp_train_stats *stats);
+/* Renders with the network at step ds_surrogate into device memory (float32
+ * transmittance [view][i][j]); with tile_tolerance > 0 every 16x16 tile is
+ * checked at 4 pixels against the exact march (params->ds, integrator,
+ * precision) and re-rendered exactly when any differs by more. */
+int xrp_surrogate_render_device(xrp_surrogate *s, const xrp_render_params *params,
+                                const xrp_view_angle *views, int n_views,
+                                double ds_surrogate, double tile_tolerance,
+                                void *out_device, xrp_surrogate_stats *stats);
+/* The same with the images copied to host memory (n_views * res * res). */
+int xrp_surrogate_render(xrp_surrogate *s, const xrp_render_params *params,
+                         const xrp_view_angle *views, int n_views, double ds_surrogate,
+                         double tile_tolerance, float *out_host, xrp_surrogate_stats *stats);
+/* Parameters (FP32, layout: tables[16][T][2], W1[32][64], b1[64], W2[64][64],
+ * b2[64], w3[64], b3) and their count. */
+int64_t xrp_surrogate_param_count(const xrp_surrogate *s);
+int xrp_surrogate_get_params(const xrp_surrogate *s, float *host);
+int xrp_surrogate_set_params(xrp_surrogate *s, const float *host);
+/* Network output at n points u in [0,1]^3 (host arrays; BF16 MFMA path). */
+int xrp_surrogate_predict(xrp_surrogate *s, const float *u_host, int64_t n, float *out_host);
+/* Mean squared error and its gradients (parameter layout) for n points and
+ * targets, computed by the training kernels (host arrays). */
+int xrp_surrogate_gradients(xrp_surrogate *s, const float *u_host, const float *targets,
+                            int64_t n, float *grads_host, double *loss);
+void xrp_surrogate_destroy(xrp_surrogate *s);
+/* Diagnostic: D = A B for 16x16 row-major float matrices through one
+ * v_mfma_f32_16x16x16_bf16 with the operand layout the surrogate uses. */
+int xrp_diag_mfma_bf16(xrp_context *ctx, const float *a, const float *b, float *d);
 
 /* ---- reference JSON API (api.go:82-180) ----------------------------------- */
 /* Renders as the reference's RenderProjections does (PNG frames,

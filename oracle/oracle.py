@@ -73,6 +73,7 @@ def lib():
             "orc_def_linear": (P, [DP]),
             "orc_def_rigid": (P, [DP]),
             "orc_def_sigmoid": (P, [D, D, D, I]),
+            "orc_def_tanh": (P, [D, D, I, I]),
             "orc_def_composed": (P, []),
             "orc_def_composed_add": (I, [P, P]),
             "orc_deform_free": (None, [P]),
@@ -422,6 +423,17 @@ def build_deformation(data: Dict, numbers: str = "yaml") -> Deform:
         if dr not in ("x", "y", "z"):
             raise ValueError("Invalid direction")
         return Deform(L.orc_def_sigmoid(amp, cen, ls, "xyz".index(dr)))
+    if t == "tanh_shear":
+        # extension (no deformations.go counterpart): p[direction] +=
+        # amplitude * tanh(rate * p[source])
+        amp = N.to_float64(data.get("amplitude"), "amplitude must be a float")
+        rate = N.to_float64(data.get("rate"), "rate must be a float")
+        dr, src = data.get("direction"), data.get("source")
+        if not isinstance(dr, str) or not isinstance(src, str):
+            raise ValueError("direction and source must be strings")
+        if dr not in ("x", "y", "z") or src not in ("x", "y", "z") or dr == src:
+            raise ValueError("Invalid direction")
+        return Deform(L.orc_def_tanh(amp, rate, "xyz".index(dr), "xyz".index(src)))
     if t == "composed":
         lst = data.get("deformations")
         if not isinstance(lst, list):

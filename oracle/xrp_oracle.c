@@ -375,7 +375,7 @@ double orc_min_feature_size(const orc_node *n) {
 }
 
 /* ---- deformations -------------------------------------------------------- */
-enum { D_GAUSS = 1, D_AFFINE, D_LINEAR, D_RIGID, D_SIGMOID, D_COMPOSED };
+enum { D_GAUSS = 1, D_AFFINE, D_LINEAR, D_RIGID, D_SIGMOID, D_COMPOSED, D_TANH };
 
 struct orc_deform {
   int type;
@@ -422,6 +422,16 @@ orc_deform *orc_def_sigmoid(double amplitude, double center, double lengthscale,
   d->p[1] = center;
   d->p[2] = lengthscale;
   d->axis = axis;
+  return d;
+}
+/* Extension without a counterpart in deformations.go (parity unpinned by
+ * the reference): p[axis] += amplitude * tanh(rate * p[source]). */
+orc_deform *orc_def_tanh(double amplitude, double rate, int axis, int source) {
+  orc_deform *d = def_new(D_TANH);
+  d->p[0] = amplitude;
+  d->p[1] = rate;
+  d->axis = axis;
+  d->q[0] = (double)source;
   return d;
 }
 orc_deform *orc_def_composed(void) { return def_new(D_COMPOSED); }
@@ -480,6 +490,11 @@ void orc_deform_apply(const orc_deform *d, double p[3]) {
     case D_SIGMOID: { /* deformations.go:210-222 */
       double q = p[d->axis];
       p[d->axis] = q + d->p[0] / (1 + exp(-(q - d->p[1]) / d->p[2]));
+      return;
+    }
+    case D_TANH: {
+      double q = p[d->axis];
+      p[d->axis] = q + d->p[0] * tanh(d->p[1] * p[(int)d->q[0]]);
       return;
     }
     case D_COMPOSED: /* deformations.go:269-274 */

@@ -49,6 +49,8 @@ orc_deform *orc_def_linear(const double strains[6]);
 orc_deform *orc_def_rigid(const double disp[3]);
 orc_deform *orc_def_sigmoid(double amplitude, double center, double lengthscale,
                             int axis);
+/* p[axis] += amplitude * tanh(rate * p[source]); no reference counterpart. */
+orc_deform *orc_def_tanh(double amplitude, double rate, int axis, int source);
 orc_deform *orc_def_composed(void);
 int orc_def_composed_add(orc_deform *c, orc_deform *child);
 void orc_deform_free(orc_deform *d);

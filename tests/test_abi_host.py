@@ -243,3 +243,24 @@ def test_raw_voxel_grid_loads_like_oracle(tmp_path):
     bad = {**obj, "resolution": [3, 4, 6]}
     ok_x, ok_o, mx, mo = try_both(bad, base_dir=str(tmp_path))
     assert not ok_x and not ok_o and mx == mo
+
+
+def test_tanh_shear_scene_rules():
+    """The extension deformation loads in both builders with the same clip box
+    (the object box widened by |amplitude| along the displaced axis)."""
+    obj = W.csg().obj
+    d = {"type": "tanh_shear", "amplitude": 0.1, "rate": 8.0, "direction": "z",
+         "source": "x"}
+    sc = X.Scene.from_dict(obj, d)
+    lo, hi, empty = sc.clip_box()
+    assert not empty
+    lo0, hi0, _ = X.Scene.from_dict(obj).clip_box()
+    assert lo[2] <= lo0[2] - 0.1 and hi[2] >= hi0[2] + 0.1
+    assert abs(lo[2] - (lo0[2] - 0.1)) < 1e-6 and abs(hi[2] - (hi0[2] + 0.1)) < 1e-6
+    assert np.allclose(lo[:2], lo0[:2], atol=1e-6) and np.allclose(hi[:2], hi0[:2], atol=1e-6)
+    O.build_deformation(d)
+    for bad in [dict(d, source="z"), dict(d, direction="w"), dict(d, rate="8")]:
+        with pytest.raises(X.XrpError):
+            X.Scene.from_dict(obj, bad)
+        with pytest.raises(ValueError):
+            O.build_deformation(bad)

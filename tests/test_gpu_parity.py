@@ -89,6 +89,7 @@ def test_example_scenes(ctx, name, deformation):
         {"type": "rigid", "displacements": [0.05, 0.0, 0.0]},
         {"type": "sigmoid", "amplitude": -0.1, "center": 0.1, "lengthscale": 0.2,
          "direction": "y"}]},
+    {"type": "tanh_shear", "amplitude": 0.1, "rate": 8.0, "direction": "z", "source": "x"},
 ])
 def test_deformations(ctx, dfm):
     w = W.csg(resolution=96, n_views=3)
@@ -193,3 +194,15 @@ def test_flat_field_and_multiplier(ctx):
     w = W.csg(resolution=64, n_views=2)
     err, st, _, _ = _compare(ctx, w.obj, w.angles, 64, w.ds, ff=0.3, dm=2.5)
     assert err <= TOL
+
+
+def test_gyroid_lattice_tanh_shear_reduced(ctx):
+    """The surrogate workload's exact scene (6^3 gyroid cells under
+    z += 0.1 tanh(8 x)) at reduced size, both integrators."""
+    w = W.gyroid_lattice(resolution=48, n_views=2, ds=0.004, shear=True)
+    for integ in ("simple", "hierarchical"):
+        err, st, _, ref = _compare(ctx, w.obj, w.angles, w.resolution, w.ds,
+                                   deformation=w.deformation, integrator=integ)
+        assert err <= TOL
+        assert st["audit_mismatch"] == 0
+        assert ref.min() < 0.95

@@ -1,0 +1,126 @@
+"""Learned density surrogate on the GPU: MFMA operand layout, forward values
+and gradients of the hand-written kernels against the FP64 restatement
+(oracle/surrogate_ref.py), training, and the surrogate march with its exact
+tile check and fallback."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import xray_projection_render_amd as X
+from xray_projection_render_amd import workloads as W
+from helpers import ROOT, xrp_scene
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import surrogate_ref as S  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(log2_table=10, n_min=4, n_max=64)
+
+
+def _small_scene():
+    w = W.gyroid_lattice(resolution=32, n_views=1, ds=0.004, shear=True)
+    return xrp_scene(w.obj, w.deformation), w
+
+
+def _random_params(rng, log2_table):
+    cfg = S.Config(log2_table=log2_table, n_min=SMALL["n_min"], n_max=SMALL["n_max"])
+    T = 1 << log2_table
+    return cfg, {
+        "tables": rng.uniform(-0.5, 0.5, (16, T, 2)),
+        "W1": rng.normal(0, 1 / math.sqrt(32), (32, 64)),
+        "b1": rng.normal(0, 0.1, 64),
+        "W2": rng.normal(0, 1 / math.sqrt(64), (64, 64)),
+        "b2": rng.normal(0, 0.1, 64),
+        "w3": rng.normal(0, 1 / math.sqrt(64), 64),
+        "b3": np.float64(0.2),
+    }
+
+
+def test_mfma_bf16_operand_layout(ctx):
+    """D = A B through v_mfma_f32_16x16x16_bf16 with the lane layout the
+    surrogate kernels assume; small integers are exact in BF16 and FP32."""
+    rng = np.random.default_rng(3)
+    a = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    b = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    d = X.diag_mfma_bf16(ctx, a, b)
+    assert np.array_equal(d, a @ b)
+
+
+def test_forward_matches_restatement(ctx):
+    sc, _ = _small_scene()
+    sur = X.Surrogate(ctx, sc, **SMALL)
+    rng = np.random.default_rng(11)
+    cfg, p = _random_params(rng, SMALL["log2_table"])
+    sur.set_params(X.join_surrogate_params(p))
+    u = rng.uniform(0, 1, (4096, 3))
+    u[:4] = [[0, 0, 0], [1, 1, 1], [0.5, 0.25, 1.0], [1e-7, 0.999999, 0.5]]
+    got = sur.predict(u).astype(np.float64)
+    want = S.predict(u.astype(np.float32).astype(np.float64), p, cfg)
+    err = np.abs(got - want)
+    print(f"forward max {err.max():.3e} mean {err.mean():.3e}", flush=True)
+    # BF16 features, weights and activations (8-bit mantissa) through two layers
+    assert err.max() < 3e-2 and err.mean() < 5e-3
+    sur.close()
+
+
+def test_gradients_match_restatement(ctx):
+    sc, _ = _small_scene()
+    sur = X.Surrogate(ctx, sc, **SMALL)
+    rng = np.random.default_rng(5)
+    cfg, p = _random_params(rng, SMALL["log2_table"])
+    sur.set_params(X.join_surrogate_params(p))
+    # 1000 samples: a partial last group and a partial last wave
+    u = rng.uniform(0, 1, (1000, 3)).astype(np.float32)
+    t = (rng.uniform(0, 1, 1000) > 0.6).astype(np.float32)
+    g, loss = sur.gradients(u, t)
+    loss_ref, g_ref = S.loss_and_grads(u.astype(np.float64), t.astype(np.float64), p, cfg)
+    assert abs(loss - loss_ref) <= 0.02 * loss_ref
+    gg = X.split_surrogate_params(g.astype(np.float64), SMALL["log2_table"])
+    for name in ("tables", "W1", "b1", "W2", "b2", "w3", "b3"):
+        a, b = np.asarray(gg[name]), np.asarray(g_ref[name])
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"{name}: rel {rel:.3e} |g| {np.linalg.norm(b):.3e}", flush=True)
+        assert rel < 0.05, name
+
+
+def test_training_and_render_with_exact_fallback(ctx):
+    """Train on exact samples of the sheared gyroid lattice, render with the
+    surrogate at twice the step with the per-tile exact check, and compare
+    the image with the exact march.  The trained network must beat the
+    untrained one by a wide margin (what the network gains), and every tile
+    whose check pixels differ by more than the tolerance is re-rendered."""
+    w = W.gyroid_lattice(resolution=64, n_views=2, ds=0.002, shear=True)
+    sc = xrp_scene(w.obj, w.deformation)
+    sur = X.Surrogate(ctx, sc, log2_table=16, n_min=8, n_max=256)
+    kw = dict(integrator="simple")
+    exact, _ = X.render(ctx, sc, w.angles, 64, w.ds, **kw)
+    img, _ = sur.render(w.angles, 64, w.ds, 2 * w.ds, 0.0, **kw)
+    err_untrained = float(np.abs(img - exact).max())
+    st = sur.train(samples=1 << 20, batch=1 << 15, epochs=6)
+    print("train", st, flush=True)
+    assert st["final_loss"] < 0.25 * st["first_loss"]
+    img, _ = sur.render(w.angles, 64, w.ds, 2 * w.ds, 0.0, **kw)
+    err_trained = float(np.abs(img - exact).max())
+    mean_trained = float(np.abs(img - exact).mean())
+    print(f"untrained max {err_untrained:.3e}; trained max {err_trained:.3e} "
+          f"mean {mean_trained:.3e}", flush=True)
+    assert err_trained < 0.5 * err_untrained
+    tol = 1e-3
+    img, rs = sur.render(w.angles, 64, w.ds, 2 * w.ds, tol, **kw)
+    print("render", rs, flush=True)
+    assert rs["tiles"] == 2 * 4 * 4 and rs["check_pixels"] == 4 * rs["tiles"]
+    # every check pixel of a tile that was not re-rendered is within tol
+    for v in range(2):
+        for ti in range(4):
+            for tj in range(4):
+                pts = [(ti * 16 + a, tj * 16 + b) for a, b in ((3, 3), (3, 11), (11, 3), (11, 11))]
+                d = max(abs(img[v, i, j] - exact[v, i, j]) for i, j in pts)
+                tile = img[v, ti * 16:(ti + 1) * 16, tj * 16:(tj + 1) * 16]
+                ref = exact[v, ti * 16:(ti + 1) * 16, tj * 16:(tj + 1) * 16]
+                assert d <= tol + 1e-6 or np.array_equal(tile, ref)
+    assert rs["max_sampled_error"] >= 0.0
+    sur.close()

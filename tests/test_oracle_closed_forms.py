@@ -251,3 +251,20 @@ def test_unit_parallelepiped_is_open_unit_box():
     o, v0, v1, v2 = map(np.array, ([0.1, 0, -0.1], [0.7, 0, 0], [0.1, 0.6, 0], [0.1, 0.1, 0.5]))
     assert sh.density(*(o + 0.3 * v0 + 0.6 * v1 + 0.2 * v2)) == -1.0
     assert sh.density(*(o + 1.1 * v0 + 0.6 * v1 + 0.2 * v2)) == 0.0
+
+
+def test_tanh_shear_closed_form():
+    """Extension deformation z += A tanh(k x): closed form at a few points,
+    identity in the other axes, odd in x."""
+    d = O.build_deformation({"type": "tanh_shear", "amplitude": 0.1, "rate": 8.0,
+                             "direction": "z", "source": "x"})
+    for p in [(0.0, 0.2, 0.3), (0.1, -0.4, 0.0), (-0.1, 0.0, -0.2), (0.5, 0.5, 0.5)]:
+        q = d.apply(*p)
+        assert q[0] == p[0] and q[1] == p[1]
+        assert abs(q[2] - (p[2] + 0.1 * math.tanh(8.0 * p[0]))) <= 1e-16
+    a = d.apply(0.07, 0.0, 0.0)
+    b = d.apply(-0.07, 0.0, 0.0)
+    assert a[2] == -b[2]
+    with pytest.raises(ValueError):
+        O.build_deformation({"type": "tanh_shear", "amplitude": 0.1, "rate": 8.0,
+                             "direction": "z", "source": "z"})

@@ -7,6 +7,7 @@ is missing the calls raise.
 from __future__ import annotations
 
 import ctypes
+import weakref
 import json
 import os
 from typing import Dict, Iterable, Optional, Sequence, Tuple
@@ -105,6 +106,38 @@ class BatchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class SurrogateConfig(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("log2_table", ctypes.c_int32),
+                ("n_min", ctypes.c_int32), ("n_max", ctypes.c_int32), ("wrap", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+
+class TrainConfig(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("batch", ctypes.c_int32),
+                ("samples", ctypes.c_int64), ("epochs", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("lr_table", ctypes.c_double),
+                ("lr_mlp", ctypes.c_double)]
+
+
+class TrainStats(ctypes.Structure):
+    _fields_ = [("first_loss", ctypes.c_double), ("final_loss", ctypes.c_double),
+                ("label_ms", ctypes.c_double), ("train_ms", ctypes.c_double),
+                ("steps", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self) -> Dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class SurrogateStats(ctypes.Structure):
+    _fields_ = [("surrogate_ms", ctypes.c_double), ("check_ms", ctypes.c_double),
+                ("fallback_ms", ctypes.c_double), ("max_sampled_error", ctypes.c_double),
+                ("tiles", ctypes.c_uint64), ("tiles_failed", ctypes.c_uint64),
+                ("check_pixels", ctypes.c_uint64)]
+
+    def as_dict(self) -> Dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
@@ -151,6 +184,22 @@ def lib() -> ctypes.CDLL:
         "xrp_batch_read": (I, [P, I, FP]),
         "xrp_batch_latencies": (I, [P, DP]),
         "xrp_batch_destroy": (None, [P]),
+        "xrp_surrogate_default_config": (None, [ctypes.POINTER(SurrogateConfig)]),
+        "xrp_train_default_config": (None, [ctypes.POINTER(TrainConfig)]),
+        "xrp_surrogate_create": (I, [P, P, ctypes.POINTER(SurrogateConfig), PP]),
+        "xrp_surrogate_train": (I, [P, ctypes.POINTER(TrainConfig), ctypes.POINTER(TrainStats)]),
+        "xrp_surrogate_render_device": (I, [P, ctypes.POINTER(RenderParams),
+                                            ctypes.POINTER(ViewAngle), I, D, D, P,
+                                            ctypes.POINTER(SurrogateStats)]),
+        "xrp_surrogate_render": (I, [P, ctypes.POINTER(RenderParams), ctypes.POINTER(ViewAngle),
+                                     I, D, D, FP, ctypes.POINTER(SurrogateStats)]),
+        "xrp_surrogate_param_count": (ctypes.c_int64, [P]),
+        "xrp_surrogate_get_params": (I, [P, FP]),
+        "xrp_surrogate_set_params": (I, [P, FP]),
+        "xrp_surrogate_predict": (I, [P, FP, ctypes.c_int64, FP]),
+        "xrp_surrogate_gradients": (I, [P, FP, FP, ctypes.c_int64, FP, DP]),
+        "xrp_surrogate_destroy": (None, [P]),
+        "xrp_diag_mfma_bf16": (I, [P, FP, FP, FP]),
         "RenderProjections": (ctypes.POINTER(ctypes.c_char), [ctypes.c_char_p]),
         "FreeString": (None, [ctypes.POINTER(ctypes.c_char)]),
     }
@@ -192,9 +241,14 @@ class Context:
         p = ctypes.c_void_p()
         check(lib().xrp_context_create(device, ctypes.byref(p)))
         self.ptr = p
+        self._owned = weakref.WeakSet()  # batches and surrogates living on it
 
     def close(self) -> None:
+        """Destroys what lives on the context (batches, surrogates), then the
+        context itself."""
         if getattr(self, "ptr", None):
+            for obj in list(getattr(self, "_owned", ())):
+                obj.close()
             lib().xrp_context_destroy(self.ptr)
             self.ptr = None
 
@@ -407,6 +461,7 @@ class Batch:
         check(lib().xrp_batch_create(ctx.ptr, arr, len(items), ctypes.byref(ptr)))
         self.ptr = ptr
         self.ctx = ctx
+        ctx._owned.add(self)
 
     def set_shape(self, wave_rows: int, wave_cols: int, groups_per_cu: int) -> None:
         t = TileShape(wave_rows, wave_cols, groups_per_cu, 0)
@@ -449,3 +504,124 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+SURROGATE_LEVELS = 16
+SURROGATE_HIDDEN = 64
+
+
+def split_surrogate_params(flat: np.ndarray, log2_table: int) -> Dict[str, np.ndarray]:
+    """Flat surrogate parameters -> named arrays (tables[16][T][2], W1[32][64],
+    b1[64], W2[64][64], b2[64], w3[64], b3)."""
+    T = 1 << log2_table
+    o = 0
+    out = {}
+    for name, shape in [("tables", (SURROGATE_LEVELS, T, 2)), ("W1", (32, 64)), ("b1", (64,)),
+                        ("W2", (64, 64)), ("b2", (64,)), ("w3", (64,)), ("b3", ())]:
+        n = int(np.prod(shape)) if shape else 1
+        out[name] = flat[o:o + n].reshape(shape) if shape else flat[o]
+        o += n
+    assert o == flat.size
+    return out
+
+
+def join_surrogate_params(p: Dict[str, np.ndarray]) -> np.ndarray:
+    return np.concatenate([np.asarray(p[k], dtype=np.float32).reshape(-1)
+                           for k in ("tables", "W1", "b1", "W2", "b2", "w3", "b3")])
+
+
+class Surrogate:
+    """Learned density surrogate of a scene (xrp_surrogate_*)."""
+
+    def __init__(self, ctx: Context, scene: Scene, log2_table: int = 19, n_min: int = 16,
+                 n_max: int = 512, wrap: int = -1, seed: int = 1):
+        c = SurrogateConfig()
+        lib().xrp_surrogate_default_config(ctypes.byref(c))
+        c.log2_table, c.n_min, c.n_max, c.wrap, c.seed = log2_table, n_min, n_max, wrap, seed
+        ptr = ctypes.c_void_p()
+        check(lib().xrp_surrogate_create(ctx.ptr, scene.ptr, ctypes.byref(c), ctypes.byref(ptr)))
+        self.ptr, self.ctx, self.scene, self.log2_table = ptr, ctx, scene, log2_table
+        ctx._owned.add(self)
+
+    def train(self, samples: int = 1 << 24, batch: int = 65536, epochs: int = 4,
+              lr_table: float = 1e-2, lr_mlp: float = 1e-3) -> Dict:
+        c = TrainConfig()
+        lib().xrp_train_default_config(ctypes.byref(c))
+        c.samples, c.batch, c.epochs, c.lr_table, c.lr_mlp = samples, batch, epochs, lr_table, lr_mlp
+        st = TrainStats()
+        check(lib().xrp_surrogate_train(self.ptr, ctypes.byref(c), ctypes.byref(st)))
+        return st.as_dict()
+
+    def render_device(self, angles, resolution: int, ds: float, ds_surrogate: float,
+                      tolerance: float, out_ptr: int, **kw) -> Dict:
+        p = make_params(resolution, ds, **kw)
+        st = SurrogateStats()
+        check(lib().xrp_surrogate_render_device(self.ptr, ctypes.byref(p), _views(angles),
+                                                len(angles), ds_surrogate, tolerance,
+                                                ctypes.c_void_p(out_ptr), ctypes.byref(st)))
+        return st.as_dict()
+
+    def render(self, angles, resolution: int, ds: float, ds_surrogate: float,
+               tolerance: float, **kw):
+        """Images [view][i][j] (float32, host) and the render stats."""
+        p = make_params(resolution, ds, **kw)
+        st = SurrogateStats()
+        out = np.empty((len(angles), resolution, resolution), dtype=np.float32)
+        check(lib().xrp_surrogate_render(self.ptr, ctypes.byref(p), _views(angles),
+                                         len(angles), ds_surrogate, tolerance,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         ctypes.byref(st)))
+        return out, st.as_dict()
+
+    def params(self) -> np.ndarray:
+        n = lib().xrp_surrogate_param_count(self.ptr)
+        out = np.empty(n, dtype=np.float32)
+        check(lib().xrp_surrogate_get_params(self.ptr, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def set_params(self, flat: np.ndarray) -> None:
+        a = np.ascontiguousarray(flat, dtype=np.float32)
+        assert a.size == lib().xrp_surrogate_param_count(self.ptr)
+        check(lib().xrp_surrogate_set_params(self.ptr, a.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float))))
+
+    def predict(self, u: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 3)
+        out = np.empty(a.shape[0], dtype=np.float32)
+        FPt = ctypes.POINTER(ctypes.c_float)
+        check(lib().xrp_surrogate_predict(self.ptr, a.ctypes.data_as(FPt), a.shape[0],
+                                          out.ctypes.data_as(FPt)))
+        return out
+
+    def gradients(self, u: np.ndarray, targets: np.ndarray):
+        a = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        g = np.empty(lib().xrp_surrogate_param_count(self.ptr), dtype=np.float32)
+        loss = ctypes.c_double()
+        FPt = ctypes.POINTER(ctypes.c_float)
+        check(lib().xrp_surrogate_gradients(self.ptr, a.ctypes.data_as(FPt),
+                                            t.ctypes.data_as(FPt), a.shape[0],
+                                            g.ctypes.data_as(FPt), ctypes.byref(loss)))
+        return g, loss.value
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().xrp_surrogate_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def diag_mfma_bf16(ctx: Context, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    FPt = ctypes.POINTER(ctypes.c_float)
+    A = np.ascontiguousarray(a, dtype=np.float32)
+    B = np.ascontiguousarray(b, dtype=np.float32)
+    D = np.empty((16, 16), dtype=np.float32)
+    check(lib().xrp_diag_mfma_bf16(ctx.ptr, A.ctypes.data_as(FPt), B.ctypes.data_as(FPt),
+                                   D.ctypes.data_as(FPt)))
+    return D

@@ -33,6 +33,7 @@ SOURCES = [
     ("march.hip", True),
     ("voxel.hip", True),
     ("batch.hip", True),
+    ("surrogate.hip", True),
     ("api.cpp", False),
     ("scene.cpp", False),
     ("camera.cpp", False),

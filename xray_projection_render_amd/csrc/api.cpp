@@ -1276,3 +1276,491 @@ void xrp_batch_destroy(xrp_batch* b) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned density surrogate.
+// ---------------------------------------------------------------------------
+#include "surrogate.hpp"
+
+struct xrp_surrogate {
+  xrp_context* ctx = nullptr;
+  xrp_scene* scene = nullptr;
+  xrp::SurNet net;
+  int log2_table = 19;
+  int64_t count = 0;        // parameters
+  int64_t table_count = 0;  // of which tables
+  DevBuf params, grads, m, v, tbf16, loss, pts, labels, scene_nodes, items, vals;
+  int adam_step = 0;
+  bool packed = false;
+  ~xrp_surrogate() {
+    for (DevBuf* b : {&params, &grads, &m, &v, &tbf16, &loss, &pts, &labels, &scene_nodes,
+                      &items, &vals})
+      b->release();
+  }
+};
+
+namespace {
+
+double rho_scale(const xrp::HNode& n) {
+  using namespace xrp;
+  if (n.kind == K_COLL) {
+    double mx = 0, sum = 0;
+    for (auto& k : n.kids) {
+      const double r = rho_scale(k);
+      mx = std::max(mx, r);
+      sum += r;
+    }
+    return n.greedy ? mx : std::min(1.0, sum);
+  }
+  if (n.kind == K_TESS) return n.kids.empty() ? 0.0 : rho_scale(n.kids[0]);
+  if (n.kind == K_VOXEL) return 1.0;
+  return std::max(0.0, n.rho);
+}
+
+// Gaussian-free deterministic initial values: tables U(-1e-4, 1e-4),
+// weights U(-sqrt(6/(in+out)), +) (Glorot), biases 0.
+void init_params(std::vector<float>& p, int log2_table, uint64_t seed) {
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 12345;
+  auto next = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const size_t t = xrp::sur_table_floats(log2_table);
+  for (size_t k = 0; k < t; ++k) p[k] = (float)((2 * next() - 1) * 1e-4);
+  size_t o = t;
+  auto dense = [&](int in, int out) {
+    const double a = std::sqrt(6.0 / (in + out));
+    for (int k = 0; k < in * out; ++k) p[o++] = (float)((2 * next() - 1) * a);
+    for (int k = 0; k < out; ++k) p[o++] = 0.0f;  // bias
+  };
+  dense(xrp::kSurIn, xrp::kSurHidden);
+  dense(xrp::kSurHidden, xrp::kSurHidden);
+  dense(xrp::kSurHidden, 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+void xrp_surrogate_default_config(xrp_surrogate_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof *c);
+  c->struct_size = sizeof *c;
+  c->log2_table = 19;
+  c->n_min = 16;
+  c->n_max = 512;
+  c->wrap = -1;
+  c->seed = 1;
+}
+
+void xrp_train_default_config(xrp_train_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof *c);
+  c->struct_size = sizeof *c;
+  c->batch = 65536;
+  c->samples = (int64_t)1 << 24;
+  c->epochs = 4;
+  c->lr_table = 1e-2;
+  c->lr_mlp = 1e-3;
+}
+
+int xrp_surrogate_create(xrp_context* ctx, const xrp_scene* scene,
+                         const xrp_surrogate_config* cfg, xrp_surrogate** out) {
+  using namespace xrp;
+  if (!ctx || !scene || !cfg || !out) return fail(XRP_ERR_NULL, "NULL argument");
+  *out = nullptr;
+  if (cfg->struct_size < sizeof(xrp_surrogate_config))
+    return fail(XRP_ERR_ARG, "config.struct_size too small");
+  if (cfg->log2_table < 4 || cfg->log2_table > 24)
+    return fail(XRP_ERR_ARG, "log2_table must be in 4..24");
+  if (cfg->n_min < 1 || cfg->n_max < cfg->n_min || cfg->n_max > 65535)
+    return fail(XRP_ERR_ARG, "need 1 <= n_min <= n_max <= 65535");
+  const Scene& s = *scene->s;
+  if (s.has_voxels) return fail(XRP_ERR_UNSUPPORTED, "surrogate of voxel scenes");
+  std::unique_ptr<xrp_surrogate> sr(new xrp_surrogate());
+  sr->ctx = ctx;
+  sr->scene = const_cast<xrp_scene*>(scene);
+  sr->log2_table = cfg->log2_table;
+  SurNet& net = sr->net;
+  std::memset(&net, 0, sizeof net);
+  net.log2_table = cfg->log2_table;
+  const double b = std::exp((std::log((double)cfg->n_max) - std::log((double)cfg->n_min)) /
+                            (kSurLevels - 1));
+  for (int l = 0; l < kSurLevels; ++l)
+    net.res[l] = (int)std::floor((double)cfg->n_min * std::pow(b, (double)l));
+  const bool tess = s.root.kind == K_TESS;
+  net.wrap = cfg->wrap < 0 ? (tess ? 1 : 0) : cfg->wrap;
+  if (net.wrap && !tess) return fail(XRP_ERR_ARG, "wrap needs a tessellated root");
+  double lo[3], ext[3];
+  if (net.wrap) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = s.root.ucb[2 * k];
+      ext[k] = s.root.ucb[2 * k + 1] - s.root.ucb[2 * k];
+    }
+    for (int k = 0; k < 6; ++k) net.bounds[k] = (float)s.root.bounds[k];
+  } else {
+    const Box3& ob = s.obj_box;
+    if (ob.empty() || !ob.finite())
+      return fail(XRP_ERR_UNSUPPORTED, "surrogate needs a bounded object");
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = ob.lo[k];
+      ext[k] = ob.hi[k] - ob.lo[k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (!(ext[k] > 0)) return fail(XRP_ERR_ARG, "degenerate surrogate domain");
+    net.lo[k] = (float)lo[k];
+    net.ext[k] = (float)ext[k];
+    net.inv_ext[k] = (float)(1.0 / ext[k]);
+  }
+  net.rho_scale = (float)std::max(1e-6, rho_scale(s.root));
+  sr->count = (int64_t)sur_param_count(cfg->log2_table);
+  sr->table_count = (int64_t)sur_table_floats(cfg->log2_table);
+  std::vector<float> host((size_t)sr->count);
+  init_params(host, cfg->log2_table, cfg->seed);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t bytes = (size_t)sr->count * sizeof(float);
+  HIPCHK(sr->params.ensure(bytes));
+  HIPCHK(sr->grads.ensure(bytes));
+  HIPCHK(sr->m.ensure(bytes));
+  HIPCHK(sr->v.ensure(bytes));
+  HIPCHK(sr->tbf16.ensure(((size_t)kSurLevels << cfg->log2_table) * sizeof(uint32_t)));
+  HIPCHK(sr->loss.ensure(sizeof(double)));
+  HIPCHK(hipMemcpy(sr->params.p, host.data(), bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(sr->m.p, 0, bytes));
+  HIPCHK(hipMemset(sr->v.p, 0, bytes));
+  HIPCHK(sr->scene_nodes.ensure(s.nodes.size() * sizeof(DevNode)));
+  HIPCHK(hipMemcpy(sr->scene_nodes.p, s.nodes.data(), s.nodes.size() * sizeof(DevNode),
+                   hipMemcpyHostToDevice));
+  net.params = (float*)sr->params.p;
+  net.table_bf16 = (uint32_t*)sr->tbf16.p;
+  *out = sr.release();
+  return XRP_OK;
+}
+
+int64_t xrp_surrogate_param_count(const xrp_surrogate* s) { return s ? s->count : 0; }
+
+int xrp_surrogate_get_params(const xrp_surrogate* s, float* host) {
+  if (!s || !host) return fail(XRP_ERR_NULL, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipMemcpy(host, s->params.p, (size_t)s->count * sizeof(float), hipMemcpyDeviceToHost));
+  return XRP_OK;
+}
+
+int xrp_surrogate_set_params(xrp_surrogate* s, const float* host) {
+  if (!s || !host) return fail(XRP_ERR_NULL, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipMemcpy(s->params.p, host, (size_t)s->count * sizeof(float), hipMemcpyHostToDevice));
+  s->packed = false;
+  return XRP_OK;
+}
+
+int xrp_surrogate_predict(xrp_surrogate* s, const float* u_host, int64_t n, float* out_host) {
+  using namespace xrp;
+  if (!s || !u_host || !out_host) return fail(XRP_ERR_NULL, "NULL argument");
+  if (n <= 0) return fail(XRP_ERR_DIMS, "n must be positive");
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
+  HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
+  HIPCHK(hipMemcpy(s->pts.p, u_host, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
+  int e = launch_sur_predict(s->net, (const float*)s->pts.p, n, (float*)s->labels.p,
+                             s->ctx->stream);
+  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  HIPCHK(hipMemcpy(out_host, s->labels.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+  return XRP_OK;
+}
+
+int xrp_surrogate_gradients(xrp_surrogate* s, const float* u_host, const float* targets,
+                            int64_t n, float* grads_host, double* loss) {
+  using namespace xrp;
+  if (!s || !u_host || !targets || !grads_host) return fail(XRP_ERR_NULL, "NULL argument");
+  if (n <= 0) return fail(XRP_ERR_DIMS, "n must be positive");
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
+  HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
+  HIPCHK(hipMemcpy(s->pts.p, u_host, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->labels.p, targets, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(s->grads.p, 0, (size_t)s->count * sizeof(float), s->ctx->stream));
+  HIPCHK(hipMemsetAsync(s->loss.p, 0, sizeof(double), s->ctx->stream));
+  int e = launch_sur_train(s->net, (const float*)s->pts.p, (const float*)s->labels.p, n,
+                           (float)(1.0 / (double)n), (float*)s->grads.p, (double*)s->loss.p,
+                           s->ctx->stream);
+  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  HIPCHK(hipMemcpy(grads_host, s->grads.p, (size_t)s->count * sizeof(float),
+                   hipMemcpyDeviceToHost));
+  double l = 0;
+  HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
+  if (loss) *loss = l / (double)n;
+  return XRP_OK;
+}
+
+int xrp_surrogate_train(xrp_surrogate* s, const xrp_train_config* cfg, xrp_train_stats* st) {
+  using namespace xrp;
+  if (!s || !cfg) return fail(XRP_ERR_NULL, "NULL argument");
+  if (cfg->struct_size < sizeof(xrp_train_config))
+    return fail(XRP_ERR_ARG, "config.struct_size too small");
+  if (cfg->batch <= 0 || cfg->samples <= 0 || cfg->epochs <= 0)
+    return fail(XRP_ERR_ARG, "batch, samples and epochs must be positive");
+  xrp_context* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n = cfg->samples;
+  HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
+  HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
+  MarchParams scene;
+  std::memset(&scene, 0, sizeof scene);
+  scene.nodes = (const DevNode*)s->scene_nodes.p;
+  hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  int e = launch_sur_labels(s->net, scene, (uint64_t)(s->adam_step + 1) * 7919ull, n,
+                            (float*)s->pts.p, (float*)s->labels.p, ctx->stream);
+  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float label_ms = 0;
+  HIPCHK(hipEventElapsedTime(&label_ms, e0, e1));
+
+  double first = NAN, epoch_loss = 0;
+  int steps = 0;
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  for (int ep = 0; ep < cfg->epochs; ++ep) {
+    HIPCHK(hipMemsetAsync(s->loss.p, 0, sizeof(double), ctx->stream));
+    for (int64_t b0 = 0; b0 < n; b0 += cfg->batch) {
+      const int64_t nb = std::min<int64_t>(cfg->batch, n - b0);
+      HIPCHK(hipMemsetAsync(s->grads.p, 0, (size_t)s->count * sizeof(float), ctx->stream));
+      e = launch_sur_train(s->net, (const float*)s->pts.p + 3 * b0,
+                           (const float*)s->labels.p + b0, nb, (float)(1.0 / (double)nb),
+                           (float*)s->grads.p, (double*)s->loss.p, ctx->stream);
+      if (!e)
+        e = launch_sur_adam((float*)s->params.p, (const float*)s->grads.p, (float*)s->m.p,
+                            (float*)s->v.p, s->count, s->table_count, (float)cfg->lr_table,
+                            (float)cfg->lr_mlp, 0.9f, 0.99f, 1e-15f, ++s->adam_step,
+                            ctx->stream);
+      if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+      ++steps;
+      if (std::isnan(first)) {
+        double l = 0;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
+        first = l / (double)nb;
+      }
+    }
+    double l = 0;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
+    epoch_loss = l / (double)n;
+  }
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float train_ms = 0;
+  HIPCHK(hipEventElapsedTime(&train_ms, e0, e1));
+  s->packed = false;
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    st->first_loss = first;
+    st->final_loss = epoch_loss;
+    st->label_ms = label_ms;
+    st->train_ms = train_ms;
+    st->steps = steps;
+  }
+  return XRP_OK;
+}
+
+int xrp_surrogate_render_device(xrp_surrogate* s, const xrp_render_params* p,
+                                const xrp_view_angle* views, int n_views, double ds_sur,
+                                double tol, void* out_device, xrp_surrogate_stats* st) {
+  using namespace xrp;
+  if (!s || !views || !out_device) return fail(XRP_ERR_NULL, "NULL argument");
+  int rc = check_params(p, n_views);
+  if (rc) return rc;
+  if (p->output_format != XRP_OUTPUT_F32)
+    return fail(XRP_ERR_ARG, "the surrogate writes float32 images");
+  if (!(ds_sur > 0) || !std::isfinite(ds_sur)) return fail(XRP_ERR_ARG, "ds_surrogate > 0");
+  if (n_views > 65535) return fail(XRP_ERR_DIMS, "at most 65535 views per call");
+  xrp_context* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  RenderJob job = make_job(p, views, n_views);
+  LaunchPlan pl;
+  rc = plan_job(ctx, s->scene, job, pl, /*allow_voxel_kernel=*/false);
+  if (rc) return rc;
+  const Scene& sc = *s->scene->s;
+  const int res = job.res;
+  // exact plan on the context buffers
+  HIPCHK(ctx->table.ensure(std::max<size_t>(1, pl.table.size()) * sizeof(double)));
+  if (!pl.table.empty())
+    HIPCHK(hipMemcpy(ctx->table.p, pl.table.data(), pl.table.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+  ctx->table_mode = -1;  // the cached table key no longer matches
+  HIPCHK(ctx->nodes.ensure(pl.nodes.size() * sizeof(DevNode)));
+  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, sc.ddefs.size()) * sizeof(DevDeform)));
+  HIPCHK(ctx->views.ensure(job.views.size() * sizeof(DevView)));
+  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
+  HIPCHK(hipMemcpy(ctx->nodes.p, pl.nodes.data(), pl.nodes.size() * sizeof(DevNode),
+                   hipMemcpyHostToDevice));
+  if (!sc.ddefs.empty())
+    HIPCHK(hipMemcpy(ctx->defs.p, sc.ddefs.data(), sc.ddefs.size() * sizeof(DevDeform),
+                     hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->views.p, job.views.data(), job.views.size() * sizeof(DevView),
+                   hipMemcpyHostToDevice));
+  MarchParams mp = pl.mp;
+  mp.nodes = (const DevNode*)ctx->nodes.p;
+  mp.defs = (const DevDeform*)ctx->defs.p;
+  mp.views = (const DevView*)ctx->views.p;
+  mp.s_table = (const double*)ctx->table.p;
+  mp.stats = nullptr;
+  mp.out = (float*)out_device;
+  mp.view_offset = 0;
+  mp.n_views = n_views;
+  if (!s->packed) {
+    int e = launch_sur_pack_tables(s->net, ctx->stream);
+    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+    s->packed = true;
+  }
+  xrp_surrogate_stats out_st;
+  std::memset(&out_st, 0, sizeof out_st);
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  int e = launch_sur_render(s->net, mp, ds_sur, ctx->stream);
+  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  out_st.surrogate_ms = ms;
+  const int tiles_side = (res + 15) / 16;
+  out_st.tiles = (uint64_t)tiles_side * tiles_side * n_views;
+  if (tol > 0) {
+    // 4 check pixels per 16x16 tile
+    std::vector<int> items;
+    items.reserve(out_st.tiles * 12);
+    const int offs[4][2] = {{3, 3}, {3, 11}, {11, 3}, {11, 11}};
+    for (int v = 0; v < n_views; ++v)
+      for (int ti = 0; ti < tiles_side; ++ti)
+        for (int tj = 0; tj < tiles_side; ++tj)
+          for (auto& o : offs) {
+            items.push_back(v);
+            items.push_back(std::min(res - 1, ti * 16 + o[0]));
+            items.push_back(std::min(res - 1, tj * 16 + o[1]));
+          }
+    const int n_check = (int)(items.size() / 3);
+    HIPCHK(s->items.ensure(items.size() * sizeof(int)));
+    HIPCHK(s->vals.ensure((size_t)n_check * sizeof(float)));
+    HIPCHK(hipMemcpy(s->items.p, items.data(), items.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+    const int prec = pl.precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED;
+    const int integ = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
+                                                                   : INT_SIMPLE;
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    e = launch_march_list(mp, prec, integ, (const int*)s->items.p, n_check,
+                          (float*)s->vals.p, ctx->stream);
+    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out_st.check_ms = ms;
+    out_st.check_pixels = (uint64_t)n_check;
+    std::vector<float> exact((size_t)n_check);
+    HIPCHK(hipMemcpy(exact.data(), s->vals.p, exact.size() * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    // the surrogate's values at the same pixels (vals is reused)
+    e = launch_gather_pixels((const float*)out_device, (const int*)s->items.p, n_check, res,
+                             (float*)s->vals.p, ctx->stream);
+    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+    std::vector<float> sur((size_t)n_check);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(sur.data(), s->vals.p, sur.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<int> fail_tiles;
+    double worst = 0;
+    for (int t = 0; t < n_check / 4; ++t) {
+      double err = 0;
+      for (int q = 0; q < 4; ++q)
+        err = std::max(err, (double)std::fabs(sur[4 * t + q] - exact[4 * t + q]));
+      worst = std::max(worst, err);
+      if (err > tol) {
+        fail_tiles.push_back(items[12 * t]);
+        fail_tiles.push_back(items[12 * t + 1] / 16);
+        fail_tiles.push_back(items[12 * t + 2] / 16);
+      }
+    }
+    out_st.max_sampled_error = worst;
+    out_st.tiles_failed = fail_tiles.size() / 3;
+    if (!fail_tiles.empty()) {
+      HIPCHK(s->items.ensure(fail_tiles.size() * sizeof(int)));
+      HIPCHK(hipMemcpy(s->items.p, fail_tiles.data(), fail_tiles.size() * sizeof(int),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+      e = launch_march_list(mp, prec, integ, (const int*)s->items.p,
+                            (int)(fail_tiles.size() / 3), nullptr, ctx->stream);
+      if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+      HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+      HIPCHK(hipEventSynchronize(ctx->ev1));
+      HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+      out_st.fallback_ms = ms;
+    }
+  }
+  if (st) *st = out_st;
+  return XRP_OK;
+}
+
+int xrp_surrogate_render(xrp_surrogate* s, const xrp_render_params* p,
+                         const xrp_view_angle* views, int n_views, double ds_sur, double tol,
+                         float* out_host, xrp_surrogate_stats* st) {
+  if (!s || !p || !out_host) return fail(XRP_ERR_NULL, "NULL argument");
+  int rc = check_params(p, n_views);
+  if (rc) return rc;
+  const size_t bytes = (size_t)n_views * p->resolution * p->resolution * sizeof(float);
+  void* dev = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    HIPCHK(hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx->out.ensure(bytes));
+    dev = s->ctx->out.p;
+  }
+  rc = xrp_surrogate_render_device(s, p, views, n_views, ds_sur, tol, dev, st);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  HIPCHK(hipMemcpy(out_host, dev, bytes, hipMemcpyDeviceToHost));
+  return XRP_OK;
+}
+
+void xrp_surrogate_destroy(xrp_surrogate* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+  }
+  delete s;
+}
+
+int xrp_diag_mfma_bf16(xrp_context* ctx, const float* a, const float* b, float* d) {
+  if (!ctx || !a || !b || !d) return fail(XRP_ERR_NULL, "NULL argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  DevBuf buf;
+  HIPCHK(buf.ensure(3 * 256 * sizeof(float)));
+  float* p = (float*)buf.p;
+  hipError_t he = hipMemcpy(p, a, 256 * sizeof(float), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(p + 256, b, 256 * sizeof(float), hipMemcpyHostToDevice);
+  int e = he == hipSuccess ? xrp::launch_mfma_probe(p, p + 256, p + 512, ctx->stream) : 0;
+  if (he == hipSuccess && e == 0) he = hipStreamSynchronize(ctx->stream);
+  if (he == hipSuccess && e == 0)
+    he = hipMemcpy(d, p + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  buf.release();
+  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+  if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+  return XRP_OK;
+}
+
+}  // extern "C"

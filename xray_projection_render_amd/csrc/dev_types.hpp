@@ -24,6 +24,7 @@ enum NodeKind : int32_t {
 
 enum DefKind : int32_t {
   D_GAUSS = 1, D_AFFINE = 2, D_LINEAR = 3, D_RIGID = 4, D_SIGMOID = 5,
+  D_TANH = 6,  // extension: p[axis] += amplitude * tanh(rate * p[source])
 };
 
 // Largest tessellation nesting the device evaluator unrolls.  Depth 0 is
@@ -65,8 +66,9 @@ static_assert(sizeof(DevNode) == 256, "DevNode must stay 256 bytes");
 
 struct alignas(16) DevDeform {
   int32_t kind;
-  int32_t axis;   // SIGMOID
-  int32_t pad[2];
+  int32_t axis;   // SIGMOID, TANH: displaced axis
+  int32_t src;    // TANH: source axis
+  int32_t pad;
   float f[12];
   double d[12];
 };
@@ -75,5 +77,6 @@ struct alignas(16) DevDeform {
 //  LINEAR f/d: strains[6]
 //  RIGID  f/d: disp[3]
 //  SIGMOID f/d: amplitude center lengthscale   (f[2] = 1/lengthscale)
+//  TANH   f/d: amplitude rate
 
 }  // namespace xrp

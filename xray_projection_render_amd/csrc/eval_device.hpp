@@ -234,6 +234,18 @@ __device__ __forceinline__ void deform64(const DevDeform* __restrict__ D,
           z = q;
         break;
       }
+      case D_TANH: {  // extension: no deformations.go counterpart
+        double s = f.src == 0 ? x : (f.src == 1 ? y : z);
+        double q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
+        q = q + f.d[0] * tanh(f.d[1] * s);
+        if (f.axis == 0)
+          x = q;
+        else if (f.axis == 1)
+          y = q;
+        else
+          z = q;
+        break;
+      }
     }
   }
 }
@@ -476,6 +488,18 @@ __device__ __forceinline__ void deform32(const DevDeform* __restrict__ D,
       case D_SIGMOID: {
         float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
         q = q + f.f[0] / (1.0f + expf(-(q - f.f[1]) * f.f[2]));
+        if (f.axis == 0)
+          x = q;
+        else if (f.axis == 1)
+          y = q;
+        else
+          z = q;
+        break;
+      }
+      case D_TANH: {
+        float s = f.src == 0 ? x : (f.src == 1 ? y : z);
+        float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
+        q = q + f.f[0] * tanhf(f.f[1] * s);
         if (f.axis == 0)
           x = q;
         else if (f.axis == 1)

@@ -39,6 +39,69 @@ __global__ __launch_bounds__(256) void march_kernel(MarchParams p) {
   flush_counts(p.stats, rc);
 }
 
+// Exact transmittance of listed pixels: item k = (view, i, j) -> vals[k].
+template <int PREC, int INTEG>
+__global__ __launch_bounds__(64) void march_pixels_kernel(MarchParams p, const int* items,
+                                                          int n, float* vals) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  RayCounts rc;
+  if (k < n) {
+    const int v = items[3 * k], i = items[3 * k + 1], j = items[3 * k + 2];
+    vals[k] = (float)exp(-march_T<PREC, INTEG>(p, v, i, j, rc));
+  }
+  flush_counts(p.stats, rc);
+}
+
+// Exact re-render of listed 16x16 tiles: item k = (view, ti, tj).
+template <int PREC, int INTEG>
+__global__ __launch_bounds__(256) void march_tiles_kernel(MarchParams p, const int* items) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int* it = items + 3 * blockIdx.x;
+  const int i = it[1] * 16 + (wave >> 1) * 8 + (lane >> 3);
+  const int j = it[2] * 16 + (wave & 1) * 8 + (lane & 7);
+  RayCounts rc;
+  if (i < p.res && j < p.res) march_ray<PREC, INTEG>(p, it[0], i, j, rc);
+  flush_counts(p.stats, rc);
+}
+
+int launch_march_list(const MarchParams& p, int precision, int integrator, const int* items,
+                      int n, float* vals, void* stream) {
+  if (n <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const bool f64 = precision == PREC_FP64, hier = integrator == INT_HIERARCHICAL;
+  if (vals) {
+    dim3 grid((n + 63) / 64), block(64);
+    if (f64 && hier)
+      hipLaunchKernelGGL((march_pixels_kernel<PREC_FP64, INT_HIERARCHICAL>), grid, block, 0, s,
+                         p, items, n, vals);
+    else if (f64)
+      hipLaunchKernelGGL((march_pixels_kernel<PREC_FP64, INT_SIMPLE>), grid, block, 0, s, p,
+                         items, n, vals);
+    else if (hier)
+      hipLaunchKernelGGL((march_pixels_kernel<PREC_GUARDED, INT_HIERARCHICAL>), grid, block, 0,
+                         s, p, items, n, vals);
+    else
+      hipLaunchKernelGGL((march_pixels_kernel<PREC_GUARDED, INT_SIMPLE>), grid, block, 0, s, p,
+                         items, n, vals);
+  } else {
+    dim3 grid(n), block(256);
+    if (f64 && hier)
+      hipLaunchKernelGGL((march_tiles_kernel<PREC_FP64, INT_HIERARCHICAL>), grid, block, 0, s, p,
+                         items);
+    else if (f64)
+      hipLaunchKernelGGL((march_tiles_kernel<PREC_FP64, INT_SIMPLE>), grid, block, 0, s, p,
+                         items);
+    else if (hier)
+      hipLaunchKernelGGL((march_tiles_kernel<PREC_GUARDED, INT_HIERARCHICAL>), grid, block, 0,
+                         s, p, items);
+    else
+      hipLaunchKernelGGL((march_tiles_kernel<PREC_GUARDED, INT_SIMPLE>), grid, block, 0, s, p,
+                         items);
+  }
+  return (int)hipGetLastError();
+}
+
 // Volume export (main.go:549-635): density() at x = i/res*2-1 (the i/N
 // convention of computeVoxel, main.go:208-214), out[k*res*res + i*res + j].
 __global__ __launch_bounds__(256) void volume_density_kernel(MarchParams p, int res,

@@ -83,6 +83,11 @@ inline bool voxel_layout_linear(int layout) {
   return layout == VOX_LINEAR || layout == VOX_LINEAR_CACHED;
 }
 int launch_voxel(const MarchParams& p, int layout, void* stream);
+// Exact march of a list: with vals, items are (view, i, j) pixels and
+// vals[k] receives the transmittance; without, items are (view, ti, tj)
+// 16x16 tiles rendered into p.out.
+int launch_march_list(const MarchParams& p, int precision, int integrator, const int* items,
+                      int n, float* vals, void* stream);
 
 // Batched persistent launch: every (scene, view, tile) item of one
 // (precision, integrator) group is taken from an atomic work counter by the

@@ -28,10 +28,10 @@ struct RayCounts {
   unsigned long long samples = 0, reeval = 0, miss = 0, fine = 0;
 };
 
-// March pixel (i, j) of view v (v counts from p.view_offset; the image is
-// written at out + v * out_view_stride) and store its transmittance.
+// Optical depth T (flat field included) of pixel (i, j) of view v (v counts
+// from p.view_offset).
 template <int PREC, int INTEG>
-__device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, int j,
+__device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, int j,
                                           RayCounts& rc) {
   const DevView& V = p.views[p.view_offset + v];
   unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0;
@@ -185,12 +185,20 @@ __device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, in
         }
       }
     }
-    store_pixel(p, v, i, j, T);
   }
   rc.samples += n_samples;
   rc.reeval += n_reeval;
   rc.miss += n_miss;
   rc.fine += n_fine;
+  return T;
+}
+
+// March pixel (i, j) of view v and store its transmittance (the image is
+// written at out + v * out_view_stride).
+template <int PREC, int INTEG>
+__device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, int j,
+                                          RayCounts& rc) {
+  store_pixel(p, v, i, j, march_T<PREC, INTEG>(p, v, i, j, rc));
 }
 
 // Wave-reduced counters into a launch's stats slots (lane 0 writes).

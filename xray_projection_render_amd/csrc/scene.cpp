@@ -430,6 +430,22 @@ void parse_deformation(const Value& d, const Num& N, std::vector<HDeform>& out,
     else
       throw SceneError("Invalid direction");
     if (tv->type != Value::String) throw SceneError("type must be a string");
+  } else if (t == "tanh_shear") {
+    // Extension without a deformations.go counterpart:
+    // p[direction] += amplitude * tanh(rate * p[source]).
+    f.kind = D_TANH;
+    f.p[0] = N.to_float64(d.get("amplitude"), "amplitude must be a float");
+    f.p[1] = N.to_float64(d.get("rate"), "rate must be a float");
+    const Value* dir = d.get("direction");
+    const Value* src = d.get("source");
+    if (!dir || dir->type != Value::String || !src || src->type != Value::String)
+      throw SceneError("direction and source must be strings");
+    auto axis_of = [](const std::string& a) {
+      return a == "x" ? 0 : (a == "y" ? 1 : (a == "z" ? 2 : -1));
+    };
+    f.axis = axis_of(dir->s);
+    f.src = axis_of(src->s);
+    if (f.axis < 0 || f.src < 0 || f.axis == f.src) throw SceneError("Invalid direction");
   } else if (t == "composed") {
     const Value* lst = d.get("deformations");
     if (!lst || lst->type != Value::Arr)
@@ -569,6 +585,13 @@ Box3 preimage(const HDeform& f, const Box3& b) {
       int a = f.axis;
       r.lo[a] = b.lo[a] - std::max(0.0, A);
       r.hi[a] = b.hi[a] - std::min(0.0, A);
+      return r;
+    }
+    case D_TANH: {  // displacement in [-|A|, |A|] along the axis
+      double A = std::fabs(f.p[0]);
+      int a = f.axis;
+      r.lo[a] = b.lo[a] - A;
+      r.hi[a] = b.hi[a] + A;
       return r;
     }
     case D_GAUSS:
@@ -807,6 +830,13 @@ void finalize_scene(Scene& s) {
         dd.f[1] = (float)f.p[1];
         dd.f[2] = (float)(1.0 / f.p[2]);
         break;
+      case D_TANH:
+        dd.src = f.src;
+        dd.d[0] = f.p[0];
+        dd.d[1] = f.p[1];
+        dd.f[0] = (float)f.p[0];
+        dd.f[1] = (float)f.p[1];
+        break;
     }
     s.ddefs.push_back(dd);
   }
@@ -981,6 +1011,14 @@ double compute_guards(Scene& s, double world_bound, double step) {
         double A = std::fabs(f.p[0]), c = std::fabs(f.p[1]), ls = std::fabs(f.p[2]);
         double arg = (B + c) / ls;
         delta = delta * (1 + A / (4 * ls)) + A * (u * arg + 8 * u) + 4 * u * (B + A);
+        B = B + A;
+        break;
+      }
+      case D_TANH: {
+        // Lipschitz |A| k; tanhf within a few ulp; FP32 rounding of the
+        // amplitude, the rate product and the add.
+        double A = std::fabs(f.p[0]), k = std::fabs(f.p[1]);
+        delta = delta * (1 + A * k) + A * k * 2 * u * B + 8 * u * A + 2 * u * (B + A);
         B = B + A;
         break;
       }

@@ -37,6 +37,7 @@ struct HNode {
 struct HDeform {
   int kind = 0;  // DefKind
   int axis = 0;
+  int src = 0;  // D_TANH source axis
   double p[9] = {0};
   double q[3] = {0};
   double r[3] = {0};

@@ -196,10 +196,15 @@ def batch(n_scenes: int = 32, max_prims: int = 200) -> List[Workload]:
     return out
 
 
+TANH_SHEAR = {"type": "tanh_shear", "amplitude": 0.1, "rate": 8.0, "direction": "z",
+              "source": "x"}
+
+
 def gyroid_lattice(resolution: int = 1024, n_views: int = 256, cells: int = 6,
-                   ds: float = 0.001) -> Workload:
+                   ds: float = 0.001, shear: bool = False) -> Workload:
     """Gyroid with one period per cell (scale = cell/(2 pi)), thickness 0.1,
-    tessellated cells^3 over [-0.5,0.5]^3."""
+    tessellated cells^3 over [-0.5,0.5]^3; with shear, the displacement
+    z += 0.1 tanh(8 x) of the surrogate workload."""
     cell = 1.0 / cells
     obj = {"type": "tessellated_obj_coll",
            "uc": {"objects": {"type": "object_collection", "objects": [
@@ -209,4 +214,5 @@ def gyroid_lattice(resolution: int = 1024, n_views: int = 256, cells: int = 6,
                   "ymax": cell / 2, "zmin": -cell / 2, "zmax": cell / 2},
            "xmin": -0.5, "xmax": 0.5, "ymin": -0.5, "ymax": 0.5, "zmin": -0.5, "zmax": 0.5}
     angles = [(360.0 * k / n_views, 90.0) for k in range(n_views)]
-    return Workload("gyroid", obj, None, angles, resolution, ds)
+    return Workload("gyroid", obj, dict(TANH_SHEAR) if shear else None, angles, resolution,
+                    ds)
