@@ -192,7 +192,8 @@ def bench_batch(args) -> int:
         samples += n
         flops += n * batch_flops_per_sample(w.obj)
     b = X.Batch(ctx, items)
-    shape = b.tune(repeats=1) if args.warmup > 0 else (1, 1, 2)
+    # the autotuner times 9 tile shapes on the whole batch: opt-in here
+    shape = b.tune(repeats=1) if args.tune else (1, 1, 2)
     for _ in range(max(0, args.warmup - 1)):
         b.run()
     kms = []
@@ -270,6 +271,82 @@ def batch_cpu_baseline(ws, every: int = 4) -> dict:
                       f"{len(ws)} scenes"}
 
 
+BF16_PEAK_TFLOPS = 2500.0
+SURROGATE_FLOPS_PER_SAMPLE = 2.0 * (32 * 64 + 64 * 64 + 64)  # MLP multiply-adds
+
+
+def bench_surrogate(args) -> int:
+    """Tessellated gyroid (6^3 cells, thickness 0.1) under z += 0.1 tanh(8x),
+    1024^2, ds 0.001: the exact guarded march against the learned surrogate
+    (2^24 exact training samples, step 2 ds, 16x16 tiles checked at 4 pixels
+    each against the exact march and re-rendered above 1e-3)."""
+    import torch
+    n_views = args.views or 16
+    w = W.gyroid_lattice(resolution=1024, n_views=n_views, ds=0.001, shear=True)
+    ctx = X.Context(0)
+    sc = X.Scene.from_dict(w.obj, w.deformation)
+    res = w.resolution
+    exact = torch.empty((n_views, res, res), dtype=torch.float32, device="cuda:0")
+    sur_img = torch.empty_like(exact)
+    kw = dict(integrator="simple", R=w.R, fov=w.fov)
+    for _ in range(max(1, args.warmup)):
+        X.render_device(ctx, sc, w.angles, res, w.ds, exact.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st_exact = X.render_device(ctx, sc, w.angles, res, w.ds, exact.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    exact_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+
+    sur = X.Surrogate(ctx, sc, log2_table=19, n_min=16, n_max=512)
+    tr = sur.train(samples=1 << 24, batch=65536, epochs=4)
+    ds_sur, tol = 2 * w.ds, 1e-3
+    for _ in range(max(1, args.warmup)):
+        sur.render_device(w.angles, res, w.ds, ds_sur, tol, sur_img.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = sur.render_device(w.angles, res, w.ds, ds_sur, tol, sur_img.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    sur_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    diff = (sur_img - exact).abs()
+    err_max, err_mean = float(diff.max()), float(diff.mean())
+    frac_over = float((diff > tol).float().mean())
+    pure = sur.render_device(w.angles, res, w.ds, ds_sur, 0.0, sur_img.data_ptr(), **kw)
+    pure_err = float((sur_img - exact).abs().max())
+    proj = w.pixels / (512.0 * 512.0)
+    sur_samples = pure["network_samples"]  # evaluations issued (each wave's step range)
+    achieved = SURROGATE_FLOPS_PER_SAMPLE * sur_samples / (pure["surrogate_ms"] / 1000.0) / 1e12
+    line = {
+        "metric": METRIC, "value": proj / (sur_ms / 1000.0), "unit": "512² projections/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sur_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 mfma (surrogate) + f32/f64 (exact check)", "data": "synthetic",
+        "config": {"workload": f"surrogate: gyroid 6^3 + tanh shear, {n_views} views x "
+                               f"{res}^2, ds={w.ds:g}, surrogate step {ds_sur:g}, tile "
+                               f"tolerance {tol:g}", "integrator": "simple"},
+        "plain": {"value": proj / (exact_ms / 1000.0), "ms_per_step": exact_ms,
+                  "kernel_ms": st_exact["kernel_ms"], "samples": st_exact["samples"],
+                  "reevaluated": st_exact["reevaluated"]},
+        "speedup_vs_plain": exact_ms / sur_ms,
+        "surrogate": {"render": st, "pure_surrogate_ms": pure["surrogate_ms"],
+                      "pure_surrogate_max_abs_error": pure_err, "train": tr},
+        "error_vs_plain": {"max_abs": err_max, "mean_abs": err_mean,
+                           "fraction_above_tolerance": frac_over, "tolerance": tol},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
+                     "traffic": measured_traffic("sur_render_kernel", "surrogate"),
+                     "kernel": "sur_render_kernel", "kernel_ms": pure["surrogate_ms"],
+                     "per_sample": f"{SURROGATE_FLOPS_PER_SAMPLE:.0f} flop (MLP)",
+                     "samples_per_launch": sur_samples},
+    }
+    print(json.dumps(line))
+    sur.close()
+    sc.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +355,8 @@ def main() -> int:
     ap.add_argument("--workload", default="voxel")
     ap.add_argument("--precision", default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--views", type=int, default=0, help="views of the surrogate workload")
+    ap.add_argument("--tune", action="store_true", help="autotune the batch tile shape first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 3 times: the reference "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -297,6 +376,8 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
 
     import torch
     w = make_workload(args.workload)

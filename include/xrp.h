@@ -304,6 +304,7 @@ typedef struct {
   uint64_t tiles;
   uint64_t tiles_failed;
   uint64_t check_pixels;
+  uint64_t network_samples; /* network evaluations issued by the surrogate march */
 } xrp_surrogate_stats;
 
 void xrp_surrogate_default_config(xrp_surrogate_config *c);

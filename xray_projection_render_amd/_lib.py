@@ -132,7 +132,7 @@ class SurrogateStats(ctypes.Structure):
     _fields_ = [("surrogate_ms", ctypes.c_double), ("check_ms", ctypes.c_double),
                 ("fallback_ms", ctypes.c_double), ("max_sampled_error", ctypes.c_double),
                 ("tiles", ctypes.c_uint64), ("tiles_failed", ctypes.c_uint64),
-                ("check_pixels", ctypes.c_uint64)]
+                ("check_pixels", ctypes.c_uint64), ("network_samples", ctypes.c_uint64)]
 
     def as_dict(self) -> Dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

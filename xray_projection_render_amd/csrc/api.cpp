@@ -1630,14 +1630,21 @@ int xrp_surrogate_render_device(xrp_surrogate* s, const xrp_render_params* p,
   }
   xrp_surrogate_stats out_st;
   std::memset(&out_st, 0, sizeof out_st);
+  HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long), ctx->stream));
+  MarchParams smp = mp;
+  smp.stats = (unsigned long long*)ctx->stats.p;
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  int e = launch_sur_render(s->net, mp, ds_sur, ctx->stream);
+  int e = launch_sur_render(s->net, smp, ds_sur, ctx->stream);
   if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ctx->ev1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   out_st.surrogate_ms = ms;
+  unsigned long long nsamp = 0;
+  HIPCHK(hipMemcpy(&nsamp, (unsigned long long*)ctx->stats.p + ST_SAMPLES, sizeof nsamp,
+                   hipMemcpyDeviceToHost));
+  out_st.network_samples = nsamp;
   const int tiles_side = (res + 15) / 16;
   out_st.tiles = (uint64_t)tiles_side * tiles_side * n_views;
   if (tol > 0) {

@@ -589,6 +589,8 @@ __global__ __launch_bounds__(256) void sur_render_kernel(SurNet net, MarchParams
     wlo = min(wlo, (long)__shfl_xor(wlo, o, 64));
     whi = max(whi, (long)__shfl_xor(whi, o, 64));
   }
+  if (p.stats && lane == 0 && whi >= wlo)
+    atomicAdd(p.stats + ST_SAMPLES, (unsigned long long)(whi - wlo + 1) * 16ull);
   float acc = 0.0f, comp = 0.0f;
   const float d0 = (float)r.d[0], d1 = (float)r.d[1], d2 = (float)r.d[2];
   const float dsf = (float)ds_sur;
