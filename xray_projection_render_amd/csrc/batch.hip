@@ -28,6 +28,8 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long lo
   atomicAdd(p, first_lane() ? v : 0ull);
 }
 
+// Registers unconstrained: the batched scenes' hot path (gyroids, up to 200
+// primitives) spills at 4 waves per SIMD and measured slower there.
 template <int PREC, int INTEG>
 __global__ __launch_bounds__(256) void batch_kernel(BatchGroup g,
                                                     const MarchParams* __restrict__ jobs) {

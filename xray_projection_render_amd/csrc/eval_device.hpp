@@ -59,7 +59,9 @@ __device__ __forceinline__ double cyl64(const DevNode& n, double x, double y,
   return d < n.d[7] ? n.d[8] : 0.0;
 }
 
-__device__ __forceinline__ double gyroid64(const DevNode& n, double x, double y,
+// out of line: FP64 sin/cos would otherwise set the register budget of
+// every kernel that can re-evaluate a sample
+__device__ __noinline__ double gyroid64(const DevNode& n, double x, double y,
                                            double z) {  // objects.go:1014-1032
   x = (x - n.d[0]) / n.d[3];
   y = (y - n.d[1]) / n.d[3];
@@ -68,7 +70,7 @@ __device__ __forceinline__ double gyroid64(const DevNode& n, double x, double y,
   return fabs(g) < n.d[4] ? n.d[5] : 0.0;
 }
 
-__device__ __forceinline__ double voxel64(const DevNode& n, double x, double y,
+__device__ __noinline__ double voxel64(const DevNode& n, double x, double y,
                                           double z) {  // objects.go:789-855
   // Same test as the reference for finite points; NaN (which the
   // reference would turn into an out-of-range index) reads nothing.
@@ -257,14 +259,6 @@ __device__ __forceinline__ double density64(const MarchParams& p, double x,
   return node64<0>(p.nodes, 0, x, y, z);
 }
 
-// The same as an out-of-line call, for the rare FP64 re-evaluation inside
-// the FP32 march: kept out of the caller, its registers do not add to the
-// pressure of the hot loop.
-__device__ __noinline__ double density64_call(const DevNode* nodes, const DevDeform* defs,
-                                              int n_defs, double x, double y, double z) {
-  deform64(defs, n_defs, x, y, z);
-  return node64<0>(nodes, 0, x, y, z);
-}
 
 // ---------------------------------------------------------------------------
 // Guarded FP32 evaluator.  Returns the exact FP64 density when every

@@ -95,8 +95,8 @@ __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, in
                 const bool a = (amb >> q) & 1u;
                 if (a || p.audit) {
                   double s = p.s_table[k0 + jb + q];
-                  double e = density64_call(p.nodes, p.defs, p.n_defs, r.o[0] + r.d[0] * s,
-                                            r.o[1] + r.d[1] * s, r.o[2] + r.d[2] * s);
+                  double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                       r.o[2] + r.d[2] * s);
                   if (a) {
                     ++n_reeval;
                   } else if (e != rho) {
