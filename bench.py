@@ -354,6 +354,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="voxel")
     ap.add_argument("--precision", default="auto")
+    ap.add_argument("--integrator", default="", help="override the workload's integrator")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--views", type=int, default=0, help="views of the surrogate workload")
     ap.add_argument("--tune", action="store_true", help="autotune the batch tile shape first")
@@ -381,6 +382,8 @@ def main() -> int:
 
     import torch
     w = make_workload(args.workload)
+    if args.integrator:
+        w.integrator = args.integrator
     ctx = X.Context(0)
     scene = (X.Scene.from_voxels(w.voxels, w.deformation) if w.voxels is not None
              else X.Scene.from_dict(w.obj, w.deformation))
