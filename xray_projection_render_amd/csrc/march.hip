@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void m
   const int i = ti * 16 + (wave >> 1) * 8 + (lane >> 3);
   const int j = tj * 16 + (wave & 1) * 8 + (lane & 7);
   RayCounts rc;
-  if (i < p.res && j < p.res) march_ray<PREC, INTEG>(p, blockIdx.y, i, j, rc);
+  march_ray<PREC, INTEG>(p, blockIdx.y, i, j, i < p.res && j < p.res, rc);
   flush_counts(p.stats, rc);
 }
 
@@ -49,10 +49,15 @@ __global__ __launch_bounds__(64) void march_pixels_kernel(MarchParams p, const i
                                                           int n, float* vals) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   RayCounts rc;
-  if (k < n) {
-    const int v = items[3 * k], i = items[3 * k + 1], j = items[3 * k + 2];
-    vals[k] = (float)exp(-march_T<PREC, INTEG>(p, v, i, j, rc));
+  const bool valid = k < n;
+  int v = 0, i = 0, j = 0;
+  if (valid) {
+    v = items[3 * k];
+    i = items[3 * k + 1];
+    j = items[3 * k + 2];
   }
+  const double T = march_T<PREC, INTEG>(p, v, i, j, valid, rc);
+  if (valid) vals[k] = (float)exp(-T);
   flush_counts(p.stats, rc);
 }
 
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(256) void march_tiles_kernel(MarchParams p, const i
   const int i = it[1] * 16 + (wave >> 1) * 8 + (lane >> 3);
   const int j = it[2] * 16 + (wave & 1) * 8 + (lane & 7);
   RayCounts rc;
-  if (i < p.res && j < p.res) march_ray<PREC, INTEG>(p, it[0], i, j, rc);
+  march_ray<PREC, INTEG>(p, it[0], i, j, i < p.res && j < p.res, rc);
   flush_counts(p.stats, rc);
 }
 

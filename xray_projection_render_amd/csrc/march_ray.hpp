@@ -29,18 +29,18 @@ struct RayCounts {
 };
 
 // Optical depth T (flat field included) of pixel (i, j) of view v (v counts
-// from p.view_offset).
-template <int PREC, int INTEG>
-__device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, int j,
-                                          RayCounts& rc) {
+// from p.view_offset), simple integrator.
+template <int PREC>
+__device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, int i, int j,
+                                                 RayCounts& rc) {
   const DevView& V = p.views[p.view_offset + v];
-  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0;
+  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0;
   double T = p.flat_field;
   {
     Ray r = make_ray(p, V, i, j);
     double t_in, t_out;
     if (clip_ray(p, r, t_in, t_out)) {
-      if constexpr (INTEG == INT_SIMPLE) {
+      {
         // main.go:144-154; s_k = s_table[k], s_{k+1} = s_k + ds.
         long k_lo = 0, k_hi = p.K - 1;
         if (p.clip) {
@@ -110,81 +110,218 @@ __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, in
             n_samples += (unsigned long long)n;
           }
         }
-      } else {
-        // main.go:159-199: coarse right-endpoint samples R[n] (s_table),
-        // fine sub-steps of DS/10 from the previous coarse point where
-        // (rho == 0) toggles.
-        const double DS = p.ds, dsf = p.ds_fine;
-        long n_lo = 0, n_hi = p.K - 1;
-        if (p.clip) {
-          double a = floor((t_in - p.smin) / DS) - 3.0;
-          double b = ceil((t_out - p.smin) / DS) + 3.0;
-          n_lo = a > 0.0 ? (long)a : 0;
-          n_hi = b < (double)(p.K - 1) ? (long)b : (long)(p.K - 1);
-        }
-        bool prev_zero = true;
-        for (long n = n_lo; n <= n_hi; ++n) {
-          double right = p.s_table[n];
-          double rho;
-          {
-            double x = r.o[0] + r.d[0] * right;
-            double y = r.o[1] + r.d[1] * right;
-            double z = r.o[2] + r.d[2] * right;
-            if constexpr (PREC == PREC_FP64) {
-              rho = density64(p, x, y, z) * p.dm;
-            } else {
-              float fx = (float)x, fy = (float)y, fz = (float)z;
-              deform32(p.defs, p.n_defs, fx, fy, fz);
-              GR g = root32(p.nodes, fx, fy, fz);
-              rho = g.v;
-              if (g.amb || p.audit) {
-                double e = density64(p, x, y, z);
-                if (g.amb)
-                  ++n_reeval;
-                else if (e != rho)
-                  ++n_miss;
-                rho = e;
-              }
-              rho = rho * p.dm;
-            }
-            ++n_samples;
-          }
-          if ((rho == 0) != prev_zero) {
-            double left = n == 0 ? p.smin : p.s_table[n - 1];
-            left += dsf;
-            while (left < right) {
-              double x = r.o[0] + r.d[0] * left;
-              double y = r.o[1] + r.d[1] * left;
-              double z = r.o[2] + r.d[2] * left;
-              double rf;
-              if constexpr (PREC == PREC_FP64) {
-                rf = density64(p, x, y, z);
-              } else {
-                float fx = (float)x, fy = (float)y, fz = (float)z;
-                deform32(p.defs, p.n_defs, fx, fy, fz);
-                GR g = root32(p.nodes, fx, fy, fz);
-                rf = g.v;
-                if (g.amb || p.audit) {
-                  double e = density64(p, x, y, z);
-                  if (g.amb)
-                    ++n_reeval;
-                  else if (e != rf)
-                    ++n_miss;
-                  rf = e;
-                }
-              }
-              T += (rf * p.dm) * dsf;
-              left += dsf;
-              ++n_fine;
-            }
-            T += rho * dsf;
-          } else {
-            T += rho * DS;
-          }
-          prev_zero = rho == 0;
-        }
       }
     }
+  }
+  rc.samples += n_samples;
+  rc.reeval += n_reeval;
+  rc.miss += n_miss;
+  return T;
+}
+
+// ---------------------------------------------------------------------------
+// Hierarchical integrator (main.go:159-199), wave-cooperative.
+//
+// Coarse samples sit at the right endpoints R[n] = s_table[n] of steps of
+// DS; where (rho == 0) toggles between R[n-1] and R[n] the reference walks
+// that step again at DS/10 (9 or 10 fine samples).  Done inside each lane,
+// a fine walk stalls the other 63 lanes of the wave for its ten
+// evaluations, and on lattices some lane toggles at most coarse samples.
+// Here a toggle is appended to a per-wave queue in LDS; once the queue is
+// half full, or every lane has finished its coarse samples, the whole wave
+// drains it, one queued toggle per lane and kBlock fine samples per node
+// visit.  Each lane links its own toggles and adds their fine sums to its
+// T in a fixed order, so the result does not depend on timing; it differs
+// from the reference's only in the FP64 summation order.
+
+constexpr int kHierCap = 512;   // queued toggles per wave
+constexpr int kHierWaves = 4;   // waves per workgroup: kernels using it are bounded to 256 lanes
+
+struct HierQueue {
+  double sum[kHierCap];     // fine sum of the step: sum of (rho * dm) * DS/10
+  int32_t n[kHierCap];      // coarse index of the step's right endpoint
+  int16_t link[kHierCap];   // the owning lane's previous entry, -1 for none
+  uint8_t owner[kHierCap];  // lane whose ray the step belongs to
+};
+
+// Densities (dm not applied) at s[q] along the ray o + s d, for the samples
+// in `live`.  Guarded: FP32 membership with guard bands over the block and
+// FP64 re-evaluation of the ambiguous samples (all of them under audit),
+// through one FP64 evaluator call site.
+template <int PREC, int B>
+__device__ __forceinline__ void eval_along(const MarchParams& p, const double (&o)[3],
+                                           const double (&d)[3], const double (&s)[B],
+                                           unsigned live, double (&rho)[B],
+                                           unsigned long long& n_reeval,
+                                           unsigned long long& n_miss) {
+  if constexpr (PREC == PREC_FP64) {
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+      rho[q] = ((live >> q) & 1u)
+                   ? density64(p, o[0] + d[0] * s[q], o[1] + d[1] * s[q], o[2] + d[2] * s[q])
+                   : 0.0;
+  } else {
+    PBlock<B> P;
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      P.x[q] = (float)(o[0] + d[0] * s[q]);
+      P.y[q] = (float)(o[1] + d[1] * s[q]);
+      P.z[q] = (float)(o[2] + d[2] * s[q]);
+    }
+    deform_blk<B>(p.defs, p.n_defs, P);
+    unsigned amb = 0;
+    root_blk<B>(p.nodes, P, live, rho, amb);
+    unsigned redo = p.audit ? live : amb;
+    while (redo) {
+      const int q = __builtin_ctz(redo);
+      redo &= redo - 1u;
+      double sq = s[0], rq = rho[0];
+#pragma unroll
+      for (int t = 1; t < B; ++t)
+        if (q == t) {
+          sq = s[t];
+          rq = rho[t];
+        }
+      const double e = density64(p, o[0] + d[0] * sq, o[1] + d[1] * sq, o[2] + d[2] * sq);
+      if ((amb >> q) & 1u)
+        ++n_reeval;
+      else if (e != rq)
+        ++n_miss;
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+        if (q == t) rho[t] = e;
+    }
+  }
+}
+
+// Optical depth T of pixel (i, j) of view v, hierarchical integrator.  Every
+// lane of the wave calls it together (lanes without a pixel pass valid =
+// false and still help drain the queue); T is meaningful for valid lanes.
+template <int PREC>
+__device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int i, int j,
+                                               bool valid, RayCounts& rc) {
+  constexpr int B = PREC == PREC_FP64 ? 1 : kBlock;
+  __shared__ HierQueue queues[kHierWaves];
+  HierQueue& Q = queues[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const double DS = p.ds, dsf = p.ds_fine, dm = p.dm;
+  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0;
+  double T = p.flat_field;
+  Ray r;
+  r.o[0] = r.o[1] = r.o[2] = 0.0;
+  r.d[0] = r.d[1] = 0.0;
+  r.d[2] = 1.0;
+  int n = 0, n_hi = -1;
+  if (valid) {
+    r = make_ray(p, p.views[p.view_offset + v], i, j);
+    double t_in, t_out;
+    if (clip_ray(p, r, t_in, t_out)) {
+      n_hi = p.K - 1;
+      if (p.clip) {
+        const double a = floor((t_in - p.smin) / DS) - 3.0;
+        const double b = ceil((t_out - p.smin) / DS) + 3.0;
+        n = a > 0.0 ? (int)a : 0;
+        n_hi = b < (double)(p.K - 1) ? (int)b : p.K - 1;
+      }
+    }
+  }
+  bool prev_zero = true;  // main.go: prev_rho starts at 0
+  int head = -1;          // this lane's newest queue entry
+  int queued = 0;         // entries in the wave's queue (same on every lane)
+  for (;;) {
+    const int nb = n;
+    unsigned tog = 0;
+    if (n <= n_hi) {
+      double s[B], rho[B];
+      unsigned live = 0;
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        s[q] = p.s_table[min(nb + q, n_hi)];
+        if (nb + q <= n_hi) live |= 1u << q;
+      }
+      eval_along<PREC, B>(p, r.o, r.d, s, live, rho, n_reeval, n_miss);
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        if (!((live >> q) & 1u)) break;
+        const double rr = rho[q] * dm;
+        const bool z = rr == 0;
+        if (z != prev_zero) {
+          tog |= 1u << q;
+          T += rr * dsf;  // the reference adds rho * DS/10 after the fine walk
+        } else {
+          T += rr * DS;
+        }
+        prev_zero = z;
+      }
+      n_samples += (unsigned long long)__builtin_popcount(live);
+      n = nb + B;
+    }
+    // Append this block's toggles in (sample, lane) order.
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const bool t = (tog >> q) & 1u;
+      const unsigned long long m = __ballot(t);
+      if (t) {
+        const int pos = queued + (int)__popcll(m & __lanemask_lt());
+        Q.n[pos] = nb + q;
+        Q.owner[pos] = (uint8_t)lane;
+        Q.link[pos] = (int16_t)head;
+        head = pos;
+      }
+      queued += (int)__popcll(m);
+    }
+    const bool more = __ballot(n <= n_hi) != 0ull;
+    if (queued > kHierCap - 64 * B || (!more && queued > 0)) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int base = 0; base < queued; base += 64) {
+        const int e = base + lane;
+        const bool has = e < queued;
+        const int src = has ? (int)Q.owner[e] : lane;
+        const int nn = has ? Q.n[e] : 0;
+        double o[3], d[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          o[c] = __shfl(r.o[c], src, 64);
+          d[c] = __shfl(r.d[c], src, 64);
+        }
+        if (has) {
+          // main.go:180-188: left = R[n-1] + ds, left += ds while left < right
+          const double right = p.s_table[nn];
+          double left = (nn == 0 ? p.smin : p.s_table[nn - 1]) + dsf;
+          double S = 0.0;
+          while (left < right) {
+            double s[B], rf[B];
+            unsigned live = 0;
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+              s[q] = left;
+              if (left < right) {
+                live |= 1u << q;
+                left += dsf;
+              }
+            }
+            eval_along<PREC, B>(p, o, d, s, live, rf, n_reeval, n_miss);
+#pragma unroll
+            for (int q = 0; q < B; ++q)
+              if ((live >> q) & 1u) S += (rf[q] * dm) * dsf;
+            n_fine += (unsigned long long)__builtin_popcount(live);
+          }
+          Q.sum[e] = S;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      while (head >= 0) {
+        T += Q.sum[head];
+        head = Q.link[head];
+      }
+      queued = 0;
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!more) break;
   }
   rc.samples += n_samples;
   rc.reeval += n_reeval;
@@ -193,12 +330,25 @@ __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, in
   return T;
 }
 
-// March pixel (i, j) of view v and store its transmittance (the image is
-// written at out + v * out_view_stride).
+// T of pixel (i, j) of view v.  Every lane of the wave calls it together;
+// lanes with valid = false have no pixel and their return value is unused.
 template <int PREC, int INTEG>
-__device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, int j,
+__device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
-  store_pixel(p, v, i, j, march_T<PREC, INTEG>(p, v, i, j, rc));
+  if constexpr (INTEG == INT_HIERARCHICAL) {
+    return march_T_hier<PREC>(p, v, i, j, valid, rc);
+  } else {
+    return valid ? march_T_simple<PREC>(p, v, i, j, rc) : 0.0;
+  }
+}
+
+// March pixel (i, j) of view v and store its transmittance (the image is
+// written at out + v * out_view_stride).  Called by every lane of the wave.
+template <int PREC, int INTEG>
+__device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, int j, bool valid,
+                                          RayCounts& rc) {
+  const double T = march_T<PREC, INTEG>(p, v, i, j, valid, rc);
+  if (valid) store_pixel(p, v, i, j, T);
 }
 
 // Wave-reduced counters into a launch's stats slots (lane 0 writes).
