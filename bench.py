@@ -192,16 +192,22 @@ def bench_batch(args) -> int:
         samples += n
         flops += n * batch_flops_per_sample(w.obj)
     b = X.Batch(ctx, items)
+    print(f"batch: {len(ws)} scenes planned", file=sys.stderr, flush=True)
     # the autotuner times 9 tile shapes on the whole batch: opt-in here
     shape = b.tune(repeats=1) if args.tune else (1, 1, 2)
     for _ in range(max(0, args.warmup - 1)):
         b.run()
+        print("batch: warm-up run done", file=sys.stderr, flush=True)
     kms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for step in range(args.steps):
         kms.append(b.run()["kernel_ms"])
+        print(f"batch: run {step + 1}/{args.steps} {kms[-1]:.0f} ms", file=sys.stderr,
+              flush=True)
     elapsed = time.perf_counter() - t0
     st = b.run()
+    print(f"batch: {args.steps} timed runs, {ms_per_step_of(elapsed, args.steps):.1f} ms each",
+          file=sys.stderr, flush=True)
     lat = b.latencies()
     pixels = sum(w.pixels for w in ws)
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -239,6 +245,10 @@ def bench_batch(args) -> int:
     return 0
 
 
+def ms_per_step_of(elapsed: float, steps: int) -> float:
+    return 1000.0 * elapsed / max(1, steps)
+
+
 def batch_cpu_baseline(ws, every: int = 4) -> dict:
     """FP64 oracle time for the whole batch, estimated on the host cores:
     one centre row per core of view 0 for every `every`-th scene, timed; the
@@ -261,6 +271,8 @@ def batch_cpu_baseline(ws, every: int = 4) -> dict:
         sc.render([az], [po], w.resolution, w.ds, R=w.R, fov=w.fov, integrator="simple",
                   nthreads=cores, rows=(r0, r0 + rows))
         spent += time.perf_counter() - t
+        print(f"cpu baseline: scene {ws.index(w)} {rows} rows of {w.resolution}^2 "
+              f"{time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
         sampled += work(w, rows * w.resolution)
     est = spent / sampled * sum(work(w, w.pixels) for w in ws)
     pixels = sum(w.pixels for w in ws)
@@ -297,9 +309,13 @@ def bench_surrogate(args) -> int:
         st_exact = X.render_device(ctx, sc, w.angles, res, w.ds, exact.data_ptr(), **kw)
     torch.cuda.synchronize()
     exact_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    print(f"surrogate bench: exact march {exact_ms:.1f} ms per step", file=sys.stderr,
+          flush=True)
 
     sur = X.Surrogate(ctx, sc, log2_table=19, n_min=16, n_max=512)
     tr = sur.train(samples=1 << 24, batch=65536, epochs=4)
+    print(f"surrogate bench: trained, loss {tr['first_loss']:.3g} -> {tr['final_loss']:.3g}",
+          file=sys.stderr, flush=True)
     ds_sur, tol = 2 * w.ds, 1e-3
     for _ in range(max(1, args.warmup)):
         sur.render_device(w.angles, res, w.ds, ds_sur, tol, sur_img.data_ptr(), **kw)

@@ -105,7 +105,8 @@ void xrp_default_params(xrp_render_params *p);
 
 typedef struct {
   uint64_t rays;
-  uint64_t samples;        /* density evaluations after bounding-box clipping */
+  uint64_t samples;        /* samples marched after bounding-box clipping (evaluated or
+                              settled by a safe radius, see skipped_samples) */
   uint64_t reevaluated;    /* guard-band samples re-evaluated in FP64 */
   uint64_t audit_mismatch; /* audit mode: FP32-clear samples that differ in FP64 */
   uint64_t fine_samples;   /* hierarchical fine sub-step evaluations */
@@ -116,6 +117,11 @@ typedef struct {
   uint64_t staged_chunks;   /* voxel kernel: sample chunks served from LDS */
   uint64_t staged_voxels;   /* voxel kernel: voxels copied into LDS */
   uint64_t fallback_chunks; /* voxel kernel: chunks gathered from global memory */
+  uint64_t skipped_samples; /* guarded simple march: samples whose density was settled
+                               without evaluation, from the safe radius (distance to
+                               the nearest boundary, less every FP32 error) of an
+                               evaluated sample before them; under audit each is
+                               also evaluated in FP64 and compared */
 } xrp_render_stats;
 
 int xrp_abi_version(void);

@@ -106,6 +106,38 @@ def test_lattice_config_reduced(ctx):
     assert st["audit_mismatch"] == 0
 
 
+@pytest.mark.parametrize("name", ["csg", "lattice", "gyroid", "random"])
+def test_safe_radius_skipping(ctx, name):
+    """The guarded simple march settles samples inside an evaluated
+    sample's safe radius without evaluating them.  With audit each settled
+    sample is also evaluated in FP64 and compared (audit_mismatch counts
+    any difference); without audit the image is the same."""
+    dfm = None
+    if name == "csg":
+        w = W.csg(resolution=96, n_views=3)
+    elif name == "lattice":
+        w = W.lattice(resolution=64, n_views=2)
+        dfm = w.deformation
+    elif name == "gyroid":
+        w = W.gyroid_lattice(resolution=48, n_views=2, ds=0.004, shear=True)
+        dfm = w.deformation
+    else:
+        obj, _, _ = W.random_scene(4, max_prims=80)
+        w = W.csg(resolution=48, n_views=2)
+        w.obj, w.ds = obj, max(xrp_scene(obj).auto_ds(), 0.004)
+    err, st, img, ref = _compare(ctx, w.obj, w.angles, w.resolution, w.ds, deformation=dfm)
+    assert err <= TOL
+    assert st["audit_mismatch"] == 0
+    assert st["precision_used"] == 1  # XRP_PRECISION_GUARDED
+    assert 0 < st["skipped_samples"] < st["samples"]
+    print(f"{name}: skipped {st['skipped_samples'] / st['samples']:.1%} of samples", flush=True)
+    sc = xrp_scene(w.obj, dfm)
+    img2, st2 = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple",
+                         audit=False)
+    assert np.array_equal(img2, img)
+    assert st2["skipped_samples"] == st["skipped_samples"]
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2, 3, 5, 8])
 def test_batch_scenes_reduced(ctx, seed):
     obj, _, _ = W.random_scene(seed, max_prims=60)

@@ -64,6 +64,7 @@ class RenderStats(ctypes.Structure):
         ("position_error", ctypes.c_double), ("precision_used", ctypes.c_int32),
         ("reserved", ctypes.c_int32), ("staged_chunks", ctypes.c_uint64),
         ("staged_voxels", ctypes.c_uint64), ("fallback_chunks", ctypes.c_uint64),
+        ("skipped_samples", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> Dict:

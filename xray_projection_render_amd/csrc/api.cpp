@@ -314,6 +314,15 @@ struct RenderJob {
 
 size_t pixel_bytes(int output) { return output == XRP_OUTPUT_GRAY8_ALPHA8 ? 2 : 4; }
 
+// $XRP_SKIP=0 turns safe-radius skipping off (read once per process).
+bool skip_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("XRP_SKIP");
+    return !(e && std::strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 // [z][x][y] grid -> 4x4x4 bricks ordered (bz, bx, by), voxels (lz, lx, ly)
 // inside a brick; padding voxels are zero and never read.
 std::vector<float> brick4(const std::vector<float>& v, int nx, int ny, int nz) {
@@ -475,10 +484,15 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   } else {
     Bw = 2.0 * std::fabs(job.R) + kCubeHalfDiagonal + 1.0;
   }
-  double delta = 0.0;
+  double delta = 0.0, lipschitz = 1.0;
   if (precision == XRP_PRECISION_GUARDED && !vox_kernel) {
     double step = job.integrator == XRP_INTEGRATE_SIMPLE ? 32.0 * job.ds : 0.0;
-    delta = compute_guards(s, Bw, step);
+    // simple march anchors: smin + k ds against the table's K accumulated
+    // additions (each off by at most half an ulp of |s|)
+    double s_err = job.integrator == XRP_INTEGRATE_SIMPLE
+                       ? (K + 4.0) * 2.3e-16 * std::max(std::fabs(smin), std::fabs(smax))
+                       : 0.0;
+    delta = compute_guards(s, Bw, step, &lipschitz, s_err);
     // The bands keep the result exact at any delta; they only cost time when
     // wide.  Fall back to FP64 when the FP32 position error reaches a tenth
     // of the smallest feature (MinFeatureSize is conservative: the gyroid's
@@ -519,6 +533,13 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   mp.out_view_stride = res * res;
   mp.out_mode = job.output;
   mp.transparency = job.transparency;
+  // Safe-radius skipping in the guarded simple march (XRP_SKIP=0 turns it
+  // off, for comparison; the image is the same either way).
+  if (precision == XRP_PRECISION_GUARDED && !vox_kernel &&
+      job.integrator == XRP_INTEGRATE_SIMPLE && skip_enabled()) {
+    double inv = 1.0 / (lipschitz * job.ds) * (1.0 - 1e-6);
+    mp.skip_inv = std::isfinite(inv) && inv > 0 ? (float)inv : 0.0f;
+  }
   mp.n_views = (int)job.views.size();
   // 4^3-brick gathers with the per-lane cell cache measured fastest on the
   // 256^3 voxel workload (4.1k projections/s; 3.0k uncached bricks, 2.8k
@@ -656,6 +677,7 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     st->staged_chunks = hst[ST_STAGED_CHUNKS];
     st->staged_voxels = hst[ST_STAGED_VOXELS];
     st->fallback_chunks = hst[ST_FALLBACK_CHUNKS];
+    st->skipped_samples = hst[ST_SKIPPED];
     st->kernel_ms = ms;
     st->position_error = delta;
     st->precision_used = precision;

@@ -70,6 +70,7 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchGroup g,
       wave_add(p.stats + ST_REEVAL, wave_sum(rc.reeval));
       wave_add(p.stats + ST_AUDIT_MISS, wave_sum(rc.miss));
       wave_add(p.stats + ST_FINE, wave_sum(rc.fine));
+      wave_add(p.stats + ST_SKIPPED, wave_sum(rc.skipped));
     }
     atomicMax(g.t_done + job, (unsigned long long)wall_clock64());
   }

@@ -10,6 +10,14 @@
 // densities are identical; a collection stops walking its children for a
 // sample exactly where the single-sample walk would (first ambiguous child,
 // or first positive child of a greedy collection).
+//
+// With DIST the walk also returns, for the block's last sample, a lower
+// bound on the distance from its object-space position to the nearest
+// boundary that could change the density there (primitive surfaces, the
+// tessellation bounds and cell seams), with every FP32 error the bound can
+// carry already subtracted (slack terms from compute_guards, g[4..6]).  Inside
+// that radius the exact density is the sample's own, which lets the march
+// settle the following samples without evaluating them.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,9 +32,9 @@ struct PBlock {
   float x[S], y[S], z[S];
 };
 
-template <int S>
+template <int S, bool DIST>
 __device__ __forceinline__ void sphere_blk(const DevNode& n, const PBlock<S>& p,
-                                           double (&v)[S], unsigned& amb) {
+                                           double (&v)[S], unsigned& amb, float& dist) {
   const float cx = n.f[0], cy = n.f[1], cz = n.f[2], r2 = n.f[3], g = n.g[0];
   const double rho = n.d[4];
 #pragma unroll
@@ -35,12 +43,19 @@ __device__ __forceinline__ void sphere_blk(const DevNode& n, const PBlock<S>& p,
     float q = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, -r2)));
     v[s] = q < 0.0f ? rho : 0.0;
     amb |= (fabsf(q) < g ? 1u : 0u) << s;
+    if constexpr (DIST) {
+      if (s == S - 1) {
+        // | |p - c| - r |
+        const float l = sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+        dist = fminf(dist, fabsf(l - n.g[4]) - n.g[5]);
+      }
+    }
   }
 }
 
-template <int S>
+template <int S, bool DIST>
 __device__ __forceinline__ void box_blk(const DevNode& n, const PBlock<S>& p,
-                                        double (&v)[S], unsigned& amb) {
+                                        double (&v)[S], unsigned& amb, float& dist) {
   const float cx = n.f[0], cy = n.f[1], cz = n.f[2], hx = n.f[3], hy = n.f[4], hz = n.f[5];
   const float g0 = n.g[0], g1 = n.g[1], g2 = n.g[2];
   const double rho = n.d[6];
@@ -53,12 +68,16 @@ __device__ __forceinline__ void box_blk(const DevNode& n, const PBlock<S>& p,
     bool in = ax < -g0 && ay < -g1 && az < -g2;
     v[s] = in ? rho : 0.0;
     amb |= (!(in || out) ? 1u : 0u) << s;
+    if constexpr (DIST) {
+      // outside: the largest face excess; inside: the nearest face
+      if (s == S - 1) dist = fminf(dist, fabsf(fmaxf(ax, fmaxf(ay, az))) - n.g[5]);
+    }
   }
 }
 
-template <int S>
+template <int S, bool DIST>
 __device__ __forceinline__ void pped_blk(const DevNode& n, const PBlock<S>& p,
-                                         double (&v)[S], unsigned& amb) {
+                                         double (&v)[S], unsigned& amb, float& dist) {
   const float ox = n.f[0], oy = n.f[1], oz = n.f[2];
   const float m0 = n.f[3], m1 = n.f[4], m2 = n.f[5], m3 = n.f[6], m4 = n.f[7], m5 = n.f[8],
               m6 = n.f[9], m7 = n.f[10], m8 = n.f[11];
@@ -76,12 +95,21 @@ __device__ __forceinline__ void pped_blk(const DevNode& n, const PBlock<S>& p,
               t < 1.0f - g2;
     v[s] = in ? rho : 0.0;
     amb |= (!(in || out) ? 1u : 0u) << s;
+    if constexpr (DIST) {
+      if (s == S - 1) {
+        // distance to the unit cube's faces in (u, v, t), less the bands,
+        // over the largest stretch of M^-1 (its Frobenius norm)
+        float e = fmaxf(fmaxf(fmaxf(-u, u - 1.0f), fmaxf(-vv, vv - 1.0f)),
+                        fmaxf(-t, t - 1.0f));
+        dist = fminf(dist, (fabsf(e) - n.g[5]) * n.g[4]);
+      }
+    }
   }
 }
 
-template <int S>
+template <int S, bool DIST>
 __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
-                                        double (&v)[S], unsigned& amb) {
+                                        double (&v)[S], unsigned& amb, float& dist) {
   const float px = n.f[0], py = n.f[1], pz = n.f[2];
   const float v0 = n.f[3], v1 = n.f[4], v2 = n.f[5], vv = n.f[6], ivv = n.f[7], r2 = n.f[8];
   const float gq = n.g[0], gt = n.g[1];
@@ -99,12 +127,21 @@ __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
     bool out = cap_out || q >= gq;
     v[s] = in ? rho : 0.0;
     amb |= (!(in || out) ? 1u : 0u) << s;
+    if constexpr (DIST) {
+      if (s == S - 1) {
+        // radial |e| - r and axial excess past the caps, both in length
+        // units; the solid's boundary is at least |max(.,.)| away
+        const float ra = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2))) - n.g[4];
+        const float aa = fmaxf(-t, t - vv) * n.g[6];
+        dist = fminf(dist, fabsf(fmaxf(ra, aa)) - n.g[5]);
+      }
+    }
   }
 }
 
-template <int S>
+template <int S, bool DIST>
 __device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
-                                           double (&v)[S], unsigned& amb) {
+                                           double (&v)[S], unsigned& amb, float& dist) {
   const float cx = n.f[0], cy = n.f[1], cz = n.f[2], sc = n.f[3], th = n.f[4], g = n.g[0];
   const double rho = n.d[5];
 #pragma unroll
@@ -118,20 +155,25 @@ __device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
     float a = fabsf(gg) - th;
     v[s] = a < 0.0f ? rho : 0.0;
     amb |= (fabsf(a) < g ? 1u : 0u) << s;
+    if constexpr (DIST) {
+      // |grad g| <= sqrt(6) / scale: the level |g| = th is at least
+      // (|a| - band) * scale / sqrt(6) away
+      if (s == S - 1) dist = fminf(dist, (fabsf(a) - g) * n.g[4]);
+    }
   }
 }
 
-template <int D, int S>
+template <int D, int S, bool DIST>
 __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
-                                         unsigned& amb_out);
+                                         unsigned& amb_out, float& dist);
 
 // Tessellation wrap (objects.go:568-582) per sample, then the unit cell's
 // collection for the samples that passed the bounds and seam guards.
-template <int D, int S>
+template <int D, int S, bool DIST>
 __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
-                                         unsigned& amb_out) {
+                                         unsigned& amb_out, float& dist) {
   const DevNode& n = N[i];
   const float b0 = n.f[0], b1 = n.f[1], b2 = n.f[2], b3 = n.f[3], b4 = n.f[4], b5 = n.f[5];
   const float gb = n.g[1], gw = n.g[0];
@@ -164,34 +206,47 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
       else
         go |= 1u << s;
     }
+    if constexpr (DIST) {
+      if (s == S - 1) {
+        // the bounds box (either side) and, inside, the cell's seams:
+        // a neighbouring cell holds the same content translated, which
+        // the unit cell's own distances do not see
+        const float eb = fmaxf(fmaxf(fmaxf(b0 - x, x - b1), fmaxf(b2 - y, y - b3)),
+                               fmaxf(b4 - z, z - b5));
+        const float sf = fminf(fminf(fminf(rx, 1.0f - rx), fminf(ry, 1.0f - ry)),
+                               fminf(rz, 1.0f - rz));
+        dist = fminf(dist, fabsf(eb) - gb);
+        if (!o) dist = fminf(dist, (sf - gw) * n.g[4]);
+      }
+    }
   }
   amb &= live;
   go &= live;
   amb_out |= amb;
   if constexpr (D + 1 <= kMaxDepth) {
-    if (go) coll_blk<D + 1, S>(N, n.first, q, go, out, amb_out);
+    if (go) coll_blk<D + 1, S, DIST>(N, n.first, q, go, out, amb_out, dist);
   } else {
     amb_out |= go;
   }
 }
 
-template <int D, int S>
+template <int D, int S, bool DIST>
 __device__ __forceinline__ void leaf_blk(const DevNode* __restrict__ N, int c,
                                          const PBlock<S>& p, unsigned live, double (&v)[S],
-                                         unsigned& amb) {
+                                         unsigned& amb, float& dist) {
   const DevNode& k = N[c];
   switch (k.kind) {
-    case K_SPHERE: sphere_blk<S>(k, p, v, amb); return;
-    case K_BOX: box_blk<S>(k, p, v, amb); return;
-    case K_PPED: pped_blk<S>(k, p, v, amb); return;
-    case K_CYL: cyl_blk<S>(k, p, v, amb); return;
-    case K_GYROID: gyroid_blk<S>(k, p, v, amb); return;
+    case K_SPHERE: sphere_blk<S, DIST>(k, p, v, amb, dist); return;
+    case K_BOX: box_blk<S, DIST>(k, p, v, amb, dist); return;
+    case K_PPED: pped_blk<S, DIST>(k, p, v, amb, dist); return;
+    case K_CYL: cyl_blk<S, DIST>(k, p, v, amb, dist); return;
+    case K_GYROID: gyroid_blk<S, DIST>(k, p, v, amb, dist); return;
     case K_TESS:
       // Tessellations below the first level (a tessellation inside a
       // tessellated cell) are left to the FP64 evaluator: the blocked
       // walk keeps one level of cell state in registers.
       if constexpr (D == 0) {
-        tess_blk<D, S>(N, c, p, live, v, amb);
+        tess_blk<D, S, DIST>(N, c, p, live, v, amb, dist);
       } else {
 #pragma unroll
         for (int s = 0; s < S; ++s) v[s] = 0.0;
@@ -199,6 +254,8 @@ __device__ __forceinline__ void leaf_blk(const DevNode* __restrict__ N, int c,
       }
       return;
     case K_VOXEL:
+      // trilinear density varies continuously: no radius
+      if constexpr (DIST) dist = 0.0f;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         GR r = voxel32(k, p.x[s], p.y[s], p.z[s]);
@@ -207,16 +264,17 @@ __device__ __forceinline__ void leaf_blk(const DevNode* __restrict__ N, int c,
       }
       return;
   }
+  if constexpr (DIST) dist = 0.0f;
 #pragma unroll
   for (int s = 0; s < S; ++s) v[s] = 0.0;
 }
 
 // objects.go:422-438 per sample: sum then clamp to [0, 1]; greedy returns
 // the first positive child unclamped.
-template <int D, int S>
+template <int D, int S, bool DIST>
 __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
-                                         unsigned& amb_out) {
+                                         unsigned& amb_out, float& dist) {
   const DevNode& n = N[i];
   const bool greedy = n.greedy;
   // out[s] holds the running sum until sample s finishes
@@ -228,7 +286,9 @@ __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
     if (!act) break;
     double v[S];
     unsigned a = 0;
-    leaf_blk<D, S>(N, c, p, act, v, a);
+    // A greedy collection's result is fixed by its children up to the
+    // first positive one, so the distances of the children walked bound it.
+    leaf_blk<D, S, DIST>(N, c, p, act, v, a, dist);
     a &= act;
     amb |= a;
     done |= a;
@@ -255,14 +315,23 @@ __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
   amb_out |= amb;
 }
 
+// dist (DIST only) starts at +inf in the caller and is lowered by every
+// node the last sample visits.
+template <int S, bool DIST = false>
+__device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PBlock<S>& p,
+                                         unsigned live, double (&out)[S], unsigned& amb,
+                                         float& dist) {
+  if (N[0].kind == K_COLL)
+    coll_blk<0, S, DIST>(N, 0, p, live, out, amb, dist);
+  else
+    leaf_blk<0, S, DIST>(N, 0, p, live, out, amb, dist);
+  amb &= live;
+}
 template <int S>
 __device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PBlock<S>& p,
                                          unsigned live, double (&out)[S], unsigned& amb) {
-  if (N[0].kind == K_COLL)
-    coll_blk<0, S>(N, 0, p, live, out, amb);
-  else
-    leaf_blk<0, S>(N, 0, p, live, out, amb);
-  amb &= live;
+  float unused = 0.0f;
+  root_blk<S, false>(N, p, live, out, amb, unused);
 }
 
 template <int S>

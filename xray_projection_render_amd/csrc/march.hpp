@@ -25,6 +25,7 @@ enum StatSlot {
   ST_STAGED_CHUNKS = 4,   // LDS voxel kernel: chunks staged in LDS
   ST_STAGED_VOXELS = 5,   //   voxels copied into LDS
   ST_FALLBACK_CHUNKS = 6, //   chunks too large for LDS (gathered)
+  ST_SKIPPED = 7,      // samples settled by a safe radius without evaluation
   ST_COUNT = 8
 };
 
@@ -66,7 +67,10 @@ struct MarchParams {
   int32_t out_mode;
   unsigned char* out8;
   int32_t transparency;
-  int32_t pad3_;
+  // Guarded simple march: 1 / (lipschitz * ds), slightly reduced; a block's
+  // last sample with safe radius r settles the next floor(r * skip_inv)
+  // samples without evaluating them.  0 disables.
+  float skip_inv;
 };
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with

@@ -25,7 +25,7 @@ constexpr int kBlock = 4;
 
 // Counters one ray adds to the launch's stats.
 struct RayCounts {
-  unsigned long long samples = 0, reeval = 0, miss = 0, fine = 0;
+  unsigned long long samples = 0, reeval = 0, miss = 0, fine = 0, skipped = 0;
 };
 
 // Optical depth T (flat field included) of pixel (i, j) of view v (v counts
@@ -34,7 +34,7 @@ template <int PREC>
 __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, int i, int j,
                                                  RayCounts& rc) {
   const DevView& V = p.views[p.view_offset + v];
-  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0;
+  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
   double T = p.flat_field;
   {
     Ray r = make_ray(p, V, i, j);
@@ -65,49 +65,84 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
         } else {
           const float ds32 = (float)ds;
           const float d0 = (float)r.d[0], d1 = (float)r.d[1], d2 = (float)r.d[2];
-          for (long k0 = k_lo; k0 <= k_hi; k0 += kChunk) {
-            double sa = p.s_table[k0];
-            float a0 = (float)(r.o[0] + r.d[0] * sa);
-            float a1 = (float)(r.o[1] + r.d[1] * sa);
-            float a2 = (float)(r.o[2] + r.d[2] * sa);
-            int n = (int)min((long)kChunk, k_hi - k0 + 1);
-            // kBlock consecutive samples per node visit (eval_block.hpp);
-            // T still adds the samples in order.
-            for (int jb = 0; jb < n; jb += kBlock) {
-              PBlock<kBlock> P;
+          // One flat loop, one block of kBlock samples per iteration, so a
+          // wave's lanes stay in step whatever each one skips.  Positions
+          // are FP32 strides of at most kChunk - 1 steps from an FP64
+          // anchor at sample ka, s_ka = smin + ka ds (within ~K ulp of the
+          // accumulated table value; the error model allows for it).
+          long k = k_lo, ka = -kChunk;
+          float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+          while (k <= k_hi) {
+            if (k + (kBlock - 1) - ka >= kChunk) {
+              ka = k;
+              const double sa = p.smin + (double)ka * ds;
+              a0 = (float)(r.o[0] + r.d[0] * sa);
+              a1 = (float)(r.o[1] + r.d[1] * sa);
+              a2 = (float)(r.o[2] + r.d[2] * sa);
+            }
+            const int j0 = (int)(k - ka);
+            PBlock<kBlock> P;
 #pragma unroll
-              for (int q = 0; q < kBlock; ++q) {
-                float sj = (float)(jb + q) * ds32;
-                P.x[q] = fmaf(d0, sj, a0);
-                P.y[q] = fmaf(d1, sj, a1);
-                P.z[q] = fmaf(d2, sj, a2);
-              }
-              deform_blk<kBlock>(p.defs, p.n_defs, P);
-              const unsigned live =
-                  n - jb >= kBlock ? (1u << kBlock) - 1u : (1u << (n - jb)) - 1u;
-              double v[kBlock];
-              unsigned amb = 0;
-              root_blk<kBlock>(p.nodes, P, live, v, amb);
+            for (int q = 0; q < kBlock; ++q) {
+              float sj = (float)(j0 + q) * ds32;
+              P.x[q] = fmaf(d0, sj, a0);
+              P.y[q] = fmaf(d1, sj, a1);
+              P.z[q] = fmaf(d2, sj, a2);
+            }
+            deform_blk<kBlock>(p.defs, p.n_defs, P);
+            const long left = k_hi - k + 1;
+            const unsigned live = left >= kBlock ? (1u << kBlock) - 1u : (1u << left) - 1u;
+            double v[kBlock];
+            unsigned amb = 0;
+            float dist = INFINITY;
+            root_blk<kBlock, true>(p.nodes, P, live, v, amb, dist);
 #pragma unroll
-              for (int q = 0; q < kBlock; ++q) {
-                if (!((live >> q) & 1u)) break;
-                double rho = v[q];
-                const bool a = (amb >> q) & 1u;
-                if (a || p.audit) {
-                  double s = p.s_table[k0 + jb + q];
-                  double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                       r.o[2] + r.d[2] * s);
-                  if (a) {
-                    ++n_reeval;
-                  } else if (e != rho) {
-                    ++n_miss;
-                  }
-                  rho = e;
+            for (int q = 0; q < kBlock; ++q) {
+              if (!((live >> q) & 1u)) break;
+              double rho = v[q];
+              const bool a = (amb >> q) & 1u;
+              if (a || p.audit) {
+                double s = p.s_table[k + q];
+                double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                     r.o[2] + r.d[2] * s);
+                if (a) {
+                  ++n_reeval;
+                } else if (e != rho) {
+                  ++n_miss;
                 }
-                T += (rho * p.dm) * ds;
+                rho = e;
+              }
+              T += (rho * p.dm) * ds;
+            }
+            const int n_live = __builtin_popcount(live);
+            n_samples += (unsigned long long)n_live;
+            k += n_live;
+            // Safe radius of the block's last sample: within dist of its
+            // object-space position the exact density is v[kBlock-1], and
+            // sample k+j lies at most lipschitz * (j+1) * ds from it.
+            if (live == (1u << kBlock) - 1u && !((amb >> (kBlock - 1)) & 1u)) {
+              const float f = dist * p.skip_inv;
+              if (f >= 1.0f && k <= k_hi) {
+                const long m = min(f < 65536.0f ? (long)f : 65536L, k_hi - k + 1);
+                const double rho_m = v[kBlock - 1];
+                if (p.audit) {
+                  for (long j = 0; j < m; ++j) {
+                    double s = p.s_table[k + j];
+                    double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                         r.o[2] + r.d[2] * s);
+                    if (e != rho_m) ++n_miss;
+                  }
+                }
+                // the terms the evaluated samples would add, in order (a
+                // zero term leaves T as it is)
+                const double c = (rho_m * p.dm) * ds;
+                if (c != 0.0)
+                  for (long j = 0; j < m; ++j) T += c;
+                n_samples += (unsigned long long)m;
+                n_skip += (unsigned long long)m;
+                k += m;
               }
             }
-            n_samples += (unsigned long long)n;
           }
         }
       }
@@ -116,6 +151,7 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
   rc.samples += n_samples;
   rc.reeval += n_reeval;
   rc.miss += n_miss;
+  rc.skipped += n_skip;
   return T;
 }
 
@@ -355,12 +391,14 @@ __device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, in
 __device__ __forceinline__ void flush_counts(unsigned long long* stats, const RayCounts& rc) {
   if (!stats) return;
   unsigned long long a = wave_sum(rc.samples), b = wave_sum(rc.reeval),
-                     c = wave_sum(rc.miss), d = wave_sum(rc.fine);
+                     c = wave_sum(rc.miss), d = wave_sum(rc.fine),
+                     e = wave_sum(rc.skipped);
   if ((threadIdx.x & 63) == 0) {
     if (a) atomicAdd(stats + ST_SAMPLES, a);
     if (b) atomicAdd(stats + ST_REEVAL, b);
     if (c) atomicAdd(stats + ST_AUDIT_MISS, c);
     if (d) atomicAdd(stats + ST_FINE, d);
+    if (e) atomicAdd(stats + ST_SKIPPED, e);
   }
 }
 

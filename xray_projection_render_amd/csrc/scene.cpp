@@ -983,11 +983,52 @@ std::unique_ptr<Scene> voxel_scene(const float* rho, int nx, int ny, int nz,
 // reference computes; B bounds |coordinate| of evaluated points.  Each band
 // is kappa times a first-order error bound of the FP32 test quantity.
 // ---------------------------------------------------------------------------
-double compute_guards(Scene& s, double world_bound, double step) {
+// Upper bound on the spectral norm of a 3x3 matrix: rho(S)^(1/2), S = M^T M,
+// is at most (||S^n||_inf)^(1/2n) for every n (an induced norm bounds the
+// spectral radius); n = 2^6 is within 3^(1/128) of the norm.
+static double spectral_norm_bound(const double* M) {
+  double S[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double a = 0;
+      for (int k = 0; k < 3; ++k) a += M[3 * k + i] * M[3 * k + j];
+      S[3 * i + j] = a;
+    }
+  double log_scale = 0.0;  // S^n = exp(log_scale) * S (renormalised)
+  int n = 1;
+  for (int it = 0; it < 6; ++it) {
+    double T[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double a = 0;
+        for (int k = 0; k < 3; ++k) a += S[3 * i + k] * S[3 * k + j];
+        T[3 * i + j] = a;
+      }
+    double mx = 0;
+    for (double t : T) mx = std::max(mx, std::fabs(t));
+    if (!(mx > 0) || !std::isfinite(mx)) return 0.0;
+    for (int k = 0; k < 9; ++k) S[k] = T[k] / mx;
+    log_scale = 2 * log_scale + std::log(mx);
+    n *= 2;
+  }
+  double row = 0;
+  for (int i = 0; i < 3; ++i)
+    row = std::max(row, std::fabs(S[3 * i]) + std::fabs(S[3 * i + 1]) + std::fabs(S[3 * i + 2]));
+  // ||S^n||_inf = exp(log_scale) * row; the bound is its 1/(2n) power
+  return std::exp((log_scale + std::log(row)) / (2.0 * n)) * (1 + 1e-9);
+}
+
+double compute_guards(Scene& s, double world_bound, double step, double* lipschitz,
+                      double s_err) {
   const double u = 1.0 / 16777216.0;
   const double kappa = 4.0;
   double B = world_bound;
-  double delta = u * (2.0 * B + 4.0 * step) + 1e-13 * (1.0 + B);
+  // FP64 terms: ray set-up, and s_err, how far the march's anchors
+  // s = smin + k ds can be from the reference's accumulated s_k.
+  double delta = u * (2.0 * B + 4.0 * step) + 1e-13 * (1.0 + B) + s_err;
+  // Euclidean Lipschitz bound of the chain: how far object-space positions
+  // of two samples can be apart per unit of ray length between them.
+  double lip_chain = 1.0;
   for (auto& f : s.defs) {
     switch (f.kind) {
       case D_RIGID: {
@@ -1000,11 +1041,15 @@ double compute_guards(Scene& s, double world_bound, double step) {
       case D_LINEAR: {
         double M[9];
         def_matrix(f, M);
-        double L = 0;
-        for (int i = 0; i < 3; ++i)
+        double L = 0, F = 0;
+        for (int i = 0; i < 3; ++i) {
           L = std::max(L, std::fabs(M[3 * i]) + std::fabs(M[3 * i + 1]) + std::fabs(M[3 * i + 2]));
+          for (int j = 0; j < 3; ++j) F += M[3 * i + j] * M[3 * i + j];
+        }
         delta = L * delta + 6 * u * L * B + 2 * u * (L + 1) * B;
         B = L * B;
+        const double sn = spectral_norm_bound(M);
+        lip_chain *= sn > 0 ? std::min(sn, std::sqrt(F)) : std::sqrt(F);
         break;
       }
       case D_SIGMOID: {
@@ -1012,6 +1057,7 @@ double compute_guards(Scene& s, double world_bound, double step) {
         double arg = (B + c) / ls;
         delta = delta * (1 + A / (4 * ls)) + A * (u * arg + 8 * u) + 4 * u * (B + A);
         B = B + A;
+        lip_chain *= 1 + A / (4 * ls);  // max slope of A / (1 + e^(-x/ls))
         break;
       }
       case D_TANH: {
@@ -1020,6 +1066,7 @@ double compute_guards(Scene& s, double world_bound, double step) {
         double A = std::fabs(f.p[0]), k = std::fabs(f.p[1]);
         delta = delta * (1 + A * k) + A * k * 2 * u * B + 8 * u * A + 2 * u * (B + A);
         B = B + A;
+        lip_chain *= 1 + A * k;
         break;
       }
       case D_GAUSS: {
@@ -1032,6 +1079,9 @@ double compute_guards(Scene& s, double world_bound, double step) {
         }
         delta = delta * (1 + lip) + 12 * u * amax + 4 * u * (B + amax);
         B = B + amax;
+        // |grad A e^(-r^2/2s^2)| <= |A| e^(-1/2) / s per component, three
+        // components: sqrt(3) * 0.6066 < 1.06
+        lip_chain *= 1 + lip;
         break;
       }
     }
@@ -1052,14 +1102,22 @@ double compute_guards(Scene& s, double world_bound, double step) {
         double cm = std::max({std::fabs(n.d[0]), std::fabs(n.d[1]), std::fabs(n.d[2])});
         double e = d + u * (cm + 1.1 * r);
         n.g[0] = (float)(kappa * (2 * s3 * 1.1 * r * e + 3 * e * e + 9 * u * r * r));
+        // distance slack: position error plus FP32 rounding of |p - c| - r
+        n.g[4] = (float)r;
+        n.g[5] = (float)(2 * d + 16 * u * (b + cm + 2 * r));
         break;
       }
-      case K_BOX:
+      case K_BOX: {
+        double hm = 0, cm = 0;
         for (int k = 0; k < 3; ++k) {
           double h = std::fabs(n.d[3 + k]);
           n.g[k] = (float)(kappa * (d + u * (std::fabs(n.d[k]) + 3 * h)));
+          hm = std::max(hm, h);
+          cm = std::max(cm, std::fabs(n.d[k]));
         }
+        n.g[5] = (float)(2 * d + 16 * u * (b + cm + hm));
         break;
+      }
       case K_PPED: {
         double om = std::max({std::fabs(n.d[0]), std::fabs(n.d[1]), std::fabs(n.d[2])});
         double W = b + om;
@@ -1072,6 +1130,11 @@ double compute_guards(Scene& s, double world_bound, double step) {
           }
           n.g[i] = (float)(kappa * (rs * d + u * ro + 6 * u * W * rs));
         }
+        // distance: (parameter-space distance - band) / ||M^-1||_F
+        double F = 0;
+        for (int k = 0; k < 9; ++k) F += n.d[3 + k] * n.d[3 + k];
+        n.g[4] = (float)(1.0 / (std::sqrt(F) * (1 + 1e-6)));
+        n.g[5] = std::max({n.g[0], n.g[1], n.g[2]}) * 2.0f;
         break;
       }
       case K_CYL: {
@@ -1083,6 +1146,10 @@ double compute_guards(Scene& s, double world_bound, double step) {
         double e = d + u * pm + 3 * u * (L + r) + L * gt / vv + 3 * u * L;
         n.g[1] = (float)(kappa * gt);
         n.g[0] = (float)(kappa * (2 * s3 * 1.1 * r * e + 3 * e * e + 9 * u * r * r));
+        // distance: radial | |e| - r | and axial t / |v| past the caps
+        n.g[4] = (float)r;
+        n.g[5] = (float)(2 * (e + gt / L) + 16 * u * (W + L + r));
+        n.g[6] = (float)(1.0 / L);
         break;
       }
       case K_GYROID: {
@@ -1091,6 +1158,9 @@ double compute_guards(Scene& s, double world_bound, double step) {
         double Q = (b + cm) / sc;
         double eq = (d + u * cm + 2 * u * (b + cm)) / sc + 2 * u * Q;
         n.g[0] = (float)(kappa * (6 * eq + 64 * u));
+        // |grad_p g| <= sqrt(6) / scale (each partial of
+        // sin x cos y + sin y cos z + sin z cos x is at most sqrt 2)
+        n.g[4] = (float)(sc / (std::sqrt(6.0) * (1 + 1e-5)));
         break;
       }
       case K_VOXEL:
@@ -1109,6 +1179,7 @@ double compute_guards(Scene& s, double world_bound, double step) {
         n.g[1] = (float)(kappa * (d + u * (bb + bm)));
         double F = (bb + um) / dxmin;
         n.g[0] = (float)(kappa * ((d + u * um + 2 * u * (bb + um)) / dxmin + 2 * u * F));
+        n.g[4] = (float)(dxmin * (1 - 1e-6));  // seam distance per unit fraction
         dl[n.depth + 1] = d + 4 * u * (bb + um + dxmax);
         Bl[n.depth + 1] = um;
         break;
@@ -1116,6 +1187,7 @@ double compute_guards(Scene& s, double world_bound, double step) {
       default: break;
     }
   }
+  if (lipschitz) *lipschitz = lip_chain;
   return delta_obj;
 }
 

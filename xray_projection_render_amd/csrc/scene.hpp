@@ -92,7 +92,11 @@ json::Value to_map(const HNode& n);
 // FP32 position error model: returns the per-component bound on
 // |p32 - p64| of object-space positions and writes guard bands into
 // s.nodes.  world_bound is max |coordinate| over sampled points, step the
-// longest FP32 stride from an FP64 anchor (chunk * ds).
-double compute_guards(Scene& s, double world_bound, double step);
+// longest FP32 stride from an FP64 anchor (chunk * ds), s_err the error of
+// the anchors' FP64 ray parameter.  Also writes the distance-bound slacks
+// (g[4..6]) of each node and, into *lipschitz, a Euclidean Lipschitz bound
+// of the deformation chain.
+double compute_guards(Scene& s, double world_bound, double step,
+                      double* lipschitz = nullptr, double s_err = 0.0);
 
 }  // namespace xrp
