@@ -117,7 +117,7 @@ typedef struct {
   uint64_t staged_chunks;   /* voxel kernel: sample chunks served from LDS */
   uint64_t staged_voxels;   /* voxel kernel: voxels copied into LDS */
   uint64_t fallback_chunks; /* voxel kernel: chunks gathered from global memory */
-  uint64_t skipped_samples; /* guarded simple march: samples whose density was settled
+  uint64_t skipped_samples; /* guarded marches: (coarse) samples whose density was settled
                                without evaluation, from the safe radius (distance to
                                the nearest boundary, less every FP32 error) of an
                                evaluated sample before them; under audit each is

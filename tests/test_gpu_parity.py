@@ -138,6 +138,30 @@ def test_safe_radius_skipping(ctx, name):
     assert st2["skipped_samples"] == st["skipped_samples"]
 
 
+@pytest.mark.parametrize("name", ["csg", "lattice"])
+def test_safe_radius_skipping_hierarchical(ctx, name):
+    """Coarse samples of the hierarchical march inside a safe radius are
+    settled without evaluation (no toggle can happen there); audit checks
+    each one against FP64."""
+    if name == "csg":
+        w, dfm = W.csg(resolution=96, n_views=3), None
+    else:
+        w = W.lattice(resolution=64, n_views=2)
+        dfm = w.deformation
+    err, st, img, _ = _compare(ctx, w.obj, w.angles, w.resolution, w.ds, deformation=dfm,
+                               integrator="hierarchical")
+    assert err <= TOL
+    assert st["audit_mismatch"] == 0
+    assert 0 < st["skipped_samples"] < st["samples"]
+    assert st["fine_samples"] > 0
+    print(f"{name}: skipped {st['skipped_samples'] / st['samples']:.1%} of coarse samples",
+          flush=True)
+    sc = xrp_scene(w.obj, dfm)
+    img2, _ = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="hierarchical",
+                       audit=False)
+    assert np.array_equal(img2, img)
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2, 3, 5, 8])
 def test_batch_scenes_reduced(ctx, seed):
     obj, _, _ = W.random_scene(seed, max_prims=60)

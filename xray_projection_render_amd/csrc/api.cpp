@@ -533,10 +533,10 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   mp.out_view_stride = res * res;
   mp.out_mode = job.output;
   mp.transparency = job.transparency;
-  // Safe-radius skipping in the guarded simple march (XRP_SKIP=0 turns it
-  // off, for comparison; the image is the same either way).
-  if (precision == XRP_PRECISION_GUARDED && !vox_kernel &&
-      job.integrator == XRP_INTEGRATE_SIMPLE && skip_enabled()) {
+  // Safe-radius skipping in the guarded marches (XRP_SKIP=0 turns it off,
+  // for comparison; the image is the same either way).  ds is the coarse
+  // step DS for the hierarchical integrator.
+  if (precision == XRP_PRECISION_GUARDED && !vox_kernel && skip_enabled()) {
     double inv = 1.0 / (lipschitz * job.ds) * (1.0 - 1e-6);
     mp.skip_inv = std::isfinite(inv) && inv > 0 ? (float)inv : 0.0f;
   }

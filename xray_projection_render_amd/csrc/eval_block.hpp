@@ -45,8 +45,9 @@ __device__ __forceinline__ void sphere_blk(const DevNode& n, const PBlock<S>& p,
     amb |= (fabsf(q) < g ? 1u : 0u) << s;
     if constexpr (DIST) {
       if (s == S - 1) {
-        // | |p - c| - r |
-        const float l = sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+        // | |p - c| - r |; the hardware square root (within an ulp, which
+        // the slack allows for) instead of the correctly rounded sequence
+        const float l = __builtin_amdgcn_sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
         dist = fminf(dist, fabsf(l - n.g[4]) - n.g[5]);
       }
     }
@@ -131,7 +132,7 @@ __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
       if (s == S - 1) {
         // radial |e| - r and axial excess past the caps, both in length
         // units; the solid's boundary is at least |max(.,.)| away
-        const float ra = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2))) - n.g[4];
+        const float ra = __builtin_amdgcn_sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2))) - n.g[4];
         const float aa = fmaxf(-t, t - vv) * n.g[6];
         dist = fminf(dist, fabsf(fmaxf(ra, aa)) - n.g[5]);
       }
@@ -156,8 +157,8 @@ __device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
     v[s] = a < 0.0f ? rho : 0.0;
     amb |= (fabsf(a) < g ? 1u : 0u) << s;
     if constexpr (DIST) {
-      // |grad g| <= sqrt(6) / scale: the level |g| = th is at least
-      // (|a| - band) * scale / sqrt(6) away
+      // |grad g| <= 1.77 / scale: the level |g| = th is at least
+      // (|a| - band) * scale / 1.77 away
       if (s == S - 1) dist = fminf(dist, (fabsf(a) - g) * n.g[4]);
     }
   }

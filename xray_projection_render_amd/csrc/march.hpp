@@ -67,9 +67,10 @@ struct MarchParams {
   int32_t out_mode;
   unsigned char* out8;
   int32_t transparency;
-  // Guarded simple march: 1 / (lipschitz * ds), slightly reduced; a block's
-  // last sample with safe radius r settles the next floor(r * skip_inv)
-  // samples without evaluating them.  0 disables.
+  // Guarded marches: 1 / (lipschitz * ds), slightly reduced (ds: the coarse
+  // step of the hierarchical integrator); a block's last sample with safe
+  // radius r settles the next floor(r * skip_inv) samples without
+  // evaluating them.  0 disables.
   float skip_inv;
 };
 

@@ -183,13 +183,16 @@ struct HierQueue {
 // Densities (dm not applied) at s[q] along the ray o + s d, for the samples
 // in `live`.  Guarded: FP32 membership with guard bands over the block and
 // FP64 re-evaluation of the ambiguous samples (all of them under audit),
-// through one FP64 evaluator call site.
-template <int PREC, int B>
+// through one FP64 evaluator call site.  With DIST (guarded only), dist is
+// the last sample's safe radius (eval_block.hpp) and last_amb whether that
+// sample was ambiguous.
+template <int PREC, int B, bool DIST = false>
 __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&o)[3],
                                            const double (&d)[3], const double (&s)[B],
                                            unsigned live, double (&rho)[B],
                                            unsigned long long& n_reeval,
-                                           unsigned long long& n_miss) {
+                                           unsigned long long& n_miss, float* dist = nullptr,
+                                           bool* last_amb = nullptr) {
   if constexpr (PREC == PREC_FP64) {
 #pragma unroll
     for (int q = 0; q < B; ++q)
@@ -206,7 +209,12 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
     }
     deform_blk<B>(p.defs, p.n_defs, P);
     unsigned amb = 0;
-    root_blk<B>(p.nodes, P, live, rho, amb);
+    if constexpr (DIST) {
+      root_blk<B, true>(p.nodes, P, live, rho, amb, *dist);
+      *last_amb = (amb >> (B - 1)) & 1u;
+    } else {
+      root_blk<B>(p.nodes, P, live, rho, amb);
+    }
     unsigned redo = p.audit ? live : amb;
     while (redo) {
       const int q = __builtin_ctz(redo);
@@ -241,7 +249,7 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
   HierQueue& Q = queues[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const double DS = p.ds, dsf = p.ds_fine, dm = p.dm;
-  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0;
+  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0, n_skip = 0;
   double T = p.flat_field;
   Ray r;
   r.o[0] = r.o[1] = r.o[2] = 0.0;
@@ -275,7 +283,11 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
         s[q] = p.s_table[min(nb + q, n_hi)];
         if (nb + q <= n_hi) live |= 1u << q;
       }
-      eval_along<PREC, B>(p, r.o, r.d, s, live, rho, n_reeval, n_miss);
+      constexpr bool kDist = PREC == PREC_GUARDED;
+      float dist = INFINITY;
+      bool last_amb = true;
+      eval_along<PREC, B, kDist>(p, r.o, r.d, s, live, rho, n_reeval, n_miss, &dist,
+                                 &last_amb);
 #pragma unroll
       for (int q = 0; q < B; ++q) {
         if (!((live >> q) & 1u)) break;
@@ -291,6 +303,29 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
       }
       n_samples += (unsigned long long)__builtin_popcount(live);
       n = nb + B;
+      // Coarse samples inside the last one's safe radius have its density:
+      // no toggle, T += rho * DS each (main.go:189-191).
+      if constexpr (kDist) {
+        const float f = dist * p.skip_inv;
+        if (live == (1u << B) - 1u && !last_amb && f >= 1.0f && n <= n_hi) {
+          const int m = min(f < 65536.0f ? (int)f : 65536, n_hi - n + 1);
+          const double rho_m = rho[B - 1];
+          if (p.audit) {
+            for (int j = 0; j < m; ++j) {
+              const double sj = p.s_table[n + j];
+              const double e = density64(p, r.o[0] + r.d[0] * sj, r.o[1] + r.d[1] * sj,
+                                         r.o[2] + r.d[2] * sj);
+              if (e != rho_m) ++n_miss;
+            }
+          }
+          const double c = (rho_m * dm) * DS;
+          if (c != 0.0)
+            for (int j = 0; j < m; ++j) T += c;
+          n_samples += (unsigned long long)m;
+          n_skip += (unsigned long long)m;
+          n += m;
+        }
+      }
     }
     // Append this block's toggles in (sample, lane) order.
 #pragma unroll
@@ -363,6 +398,7 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
   rc.reeval += n_reeval;
   rc.miss += n_miss;
   rc.fine += n_fine;
+  rc.skipped += n_skip;
   return T;
 }
 

@@ -1158,9 +1158,12 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         double Q = (b + cm) / sc;
         double eq = (d + u * cm + 2 * u * (b + cm)) / sc + 2 * u * Q;
         n.g[0] = (float)(kappa * (6 * eq + 64 * u));
-        // |grad_p g| <= sqrt(6) / scale (each partial of
-        // sin x cos y + sin y cos z + sin z cos x is at most sqrt 2)
-        n.g[4] = (float)(sc / (std::sqrt(6.0) * (1 + 1e-5)));
+        // |grad g| of g = sin x cos y + sin y cos z + sin z cos x is at most
+        // 1.77 (in q = p / scale units): |grad g|^2 peaks at 3 on a 721^3
+        // grid of the 2 pi period, and between grid points it grows by at
+        // most |grad |grad g|^2| * h sqrt(3)/2 <= 2 |H|_F |grad g| * 0.00756
+        // <= 2 sqrt(12) sqrt(6) * 0.00756 = 0.128; sqrt(3.128) < 1.77.
+        n.g[4] = (float)(sc / (1.77 * (1 + 1e-5)));
         break;
       }
       case K_VOXEL:
