@@ -26,11 +26,13 @@
 
 namespace xrp {
 
-// 4 waves per SIMD (<= 128 VGPRs): measured fastest on the lattice and CSG
-// workloads (+62 % / +12 % over the unconstrained 2 waves); what spills is
-// the rare FP64 re-evaluation path.
+// Simple integrator: 4 waves per SIMD (<= 128 VGPRs), measured fastest on
+// the lattice and CSG workloads (+62 % / +12 % over the unconstrained 2
+// waves; 3 waves: -4 % / -8 %).  Hierarchical: 3 waves (168 VGPRs, the
+// toggle queue's state stays in registers), 7 % faster than 4 on the lattice.
 template <int PREC, int INTEG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void march_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    INTEG == INT_HIERARCHICAL ? 3 : 4))) void march_kernel(
     MarchParams p) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;

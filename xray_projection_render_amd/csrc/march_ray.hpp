@@ -34,7 +34,8 @@ template <int PREC>
 __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, int i, int j,
                                                  RayCounts& rc) {
   const DevView& V = p.views[p.view_offset + v];
-  unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
+  // per-ray counts fit 32 bits (a ray has fewer than 2^31 samples)
+  unsigned n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
   double T = p.flat_field;
   {
     Ray r = make_ray(p, V, i, j);
@@ -42,25 +43,25 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
     if (clip_ray(p, r, t_in, t_out)) {
       {
         // main.go:144-154; s_k = s_table[k], s_{k+1} = s_k + ds.
-        long k_lo = 0, k_hi = p.K - 1;
+        int k_lo = 0, k_hi = p.K - 1;
         if (p.clip) {
           double a = floor((t_in - p.smin) / p.ds) - 2.0;
           double b = ceil((t_out - p.smin) / p.ds) + 2.0;
-          k_lo = a > 0.0 ? (long)a : 0;
-          k_hi = b < (double)(p.K - 1) ? (long)b : (long)(p.K - 1);
+          k_lo = a > 0.0 ? (a < (double)p.K ? (int)a : p.K) : 0;
+          k_hi = b < (double)(p.K - 1) ? (b > -1.0 ? (int)b : -1) : p.K - 1;
         }
         const double ds = p.ds;
         if constexpr (PREC == PREC_FP64) {
           if (k_lo <= k_hi) {
             double s = p.s_table[k_lo];
-            for (long k = k_lo; k <= k_hi; ++k) {
+            for (int k = k_lo; k <= k_hi; ++k) {
               double x = r.o[0] + r.d[0] * s;
               double y = r.o[1] + r.d[1] * s;
               double z = r.o[2] + r.d[2] * s;
               T += (density64(p, x, y, z) * p.dm) * ds;
               s += ds;
             }
-            n_samples += (unsigned long long)(k_hi - k_lo + 1);
+            n_samples += (unsigned)(k_hi - k_lo + 1);
           }
         } else {
           const float ds32 = (float)ds;
@@ -70,7 +71,7 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
           // are FP32 strides of at most kChunk - 1 steps from an FP64
           // anchor at sample ka, s_ka = smin + ka ds (within ~K ulp of the
           // accumulated table value; the error model allows for it).
-          long k = k_lo, ka = -kChunk;
+          int k = k_lo, ka = -kChunk;
           float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
           while (k <= k_hi) {
             if (k + (kBlock - 1) - ka >= kChunk) {
@@ -90,43 +91,50 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
               P.z[q] = fmaf(d2, sj, a2);
             }
             deform_blk<kBlock>(p.defs, p.n_defs, P);
-            const long left = k_hi - k + 1;
+            const int left = k_hi - k + 1;
             const unsigned live = left >= kBlock ? (1u << kBlock) - 1u : (1u << left) - 1u;
             double v[kBlock];
             unsigned amb = 0;
             float dist = INFINITY;
             root_blk<kBlock, true>(p.nodes, P, live, v, amb, dist);
+            // FP64 re-evaluation of the ambiguous samples (all of them under
+            // audit) through one call site, then T in sample order.
+            const bool last_amb = (amb >> (kBlock - 1)) & 1u;
+            unsigned redo = p.audit ? live : amb;
+            while (redo) {
+              const int q = __builtin_ctz(redo);
+              redo &= redo - 1u;
+              double vq = v[0];
 #pragma unroll
-            for (int q = 0; q < kBlock; ++q) {
-              if (!((live >> q) & 1u)) break;
-              double rho = v[q];
-              const bool a = (amb >> q) & 1u;
-              if (a || p.audit) {
-                double s = p.s_table[k + q];
-                double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                     r.o[2] + r.d[2] * s);
-                if (a) {
-                  ++n_reeval;
-                } else if (e != rho) {
-                  ++n_miss;
-                }
-                rho = e;
-              }
-              T += (rho * p.dm) * ds;
+              for (int t = 1; t < kBlock; ++t)
+                if (q == t) vq = v[t];
+              const double s = p.s_table[k + q];
+              const double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                         r.o[2] + r.d[2] * s);
+              if ((amb >> q) & 1u)
+                ++n_reeval;
+              else if (e != vq)
+                ++n_miss;
+#pragma unroll
+              for (int t = 0; t < kBlock; ++t)
+                if (q == t) v[t] = e;
             }
+#pragma unroll
+            for (int q = 0; q < kBlock; ++q)
+              if ((live >> q) & 1u) T += (v[q] * p.dm) * ds;
             const int n_live = __builtin_popcount(live);
-            n_samples += (unsigned long long)n_live;
+            n_samples += (unsigned)n_live;
             k += n_live;
             // Safe radius of the block's last sample: within dist of its
             // object-space position the exact density is v[kBlock-1], and
             // sample k+j lies at most lipschitz * (j+1) * ds from it.
-            if (live == (1u << kBlock) - 1u && !((amb >> (kBlock - 1)) & 1u)) {
+            if (live == (1u << kBlock) - 1u && !last_amb) {
               const float f = dist * p.skip_inv;
               if (f >= 1.0f && k <= k_hi) {
-                const long m = min(f < 65536.0f ? (long)f : 65536L, k_hi - k + 1);
+                const int m = min(f < 65536.0f ? (int)f : 65536, k_hi - k + 1);
                 const double rho_m = v[kBlock - 1];
                 if (p.audit) {
-                  for (long j = 0; j < m; ++j) {
+                  for (int j = 0; j < m; ++j) {
                     double s = p.s_table[k + j];
                     double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
                                          r.o[2] + r.d[2] * s);
@@ -137,9 +145,9 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
                 // zero term leaves T as it is)
                 const double c = (rho_m * p.dm) * ds;
                 if (c != 0.0)
-                  for (long j = 0; j < m; ++j) T += c;
-                n_samples += (unsigned long long)m;
-                n_skip += (unsigned long long)m;
+                  for (int j = 0; j < m; ++j) T += c;
+                n_samples += (unsigned)m;
+                n_skip += (unsigned)m;
                 k += m;
               }
             }
