@@ -446,6 +446,11 @@ def main() -> int:
         roof["kernel"] = "march_kernel<%s,%s>" % (
             "PREC_FP64" if prec == "fp64" else "PREC_GUARDED",
             "INT_SIMPLE" if w.integrator == "simple" else "INT_HIERARCHICAL")
+        if prec != "fp64" and w.integrator == "simple":
+            # samples per node visit the library picks: 4 from 24 nodes up, else 2
+            block = 4 if scene.layout()[0] >= 24 else 2
+            roof["kernel"] = f"march_kernel<PREC_GUARDED,INT_SIMPLE,{block}>"
+    roof["skipped_samples"] = st.get("skipped_samples", 0)
     roof["kernel_ms"] = k_ms
     roof["per_sample"] = (f"{BYTES_PER_SAMPLE[args.workload]:.0f} B" if args.workload
                           in BYTES_PER_SAMPLE else f"{FLOPS_PER_SAMPLE[args.workload]:.0f} flop")

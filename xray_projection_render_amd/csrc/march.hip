@@ -30,7 +30,7 @@ namespace xrp {
 // the lattice and CSG workloads (+62 % / +12 % over the unconstrained 2
 // waves; 3 waves: -4 % / -8 %).  Hierarchical: 3 waves (168 VGPRs, the
 // toggle queue's state stays in registers), 7 % faster than 4 on the lattice.
-template <int PREC, int INTEG>
+template <int PREC, int INTEG, int B>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     INTEG == INT_HIERARCHICAL ? 3 : 4))) void march_kernel(
     MarchParams p) {
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   const int i = ti * 16 + (wave >> 1) * 8 + (lane >> 3);
   const int j = tj * 16 + (wave & 1) * 8 + (lane & 7);
   RayCounts rc;
-  march_ray<PREC, INTEG>(p, blockIdx.y, i, j, i < p.res && j < p.res, rc);
+  march_ray<PREC, INTEG, B>(p, blockIdx.y, i, j, i < p.res && j < p.res, rc);
   flush_counts(p.stats, rc);
 }
 
@@ -173,17 +173,20 @@ int launch_march(const MarchParams& p, int precision, int integrator,
   hipStream_t s = (hipStream_t)stream;
   if (precision == PREC_FP64) {
     if (integrator == INT_HIERARCHICAL)
-      hipLaunchKernelGGL((march_kernel<PREC_FP64, INT_HIERARCHICAL>), grid,
+      hipLaunchKernelGGL((march_kernel<PREC_FP64, INT_HIERARCHICAL, kBlock>), grid,
                          block, 0, s, p);
     else
-      hipLaunchKernelGGL((march_kernel<PREC_FP64, INT_SIMPLE>), grid, block, 0,
+      hipLaunchKernelGGL((march_kernel<PREC_FP64, INT_SIMPLE, kBlock>), grid, block, 0,
                          s, p);
   } else {
     if (integrator == INT_HIERARCHICAL)
-      hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_HIERARCHICAL>), grid,
+      hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_HIERARCHICAL, kBlock>), grid,
                          block, 0, s, p);
+    else if (p.block == 2)
+      hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, 2>), grid, block,
+                         0, s, p);
     else
-      hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE>), grid, block,
+      hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, kBlock>), grid, block,
                          0, s, p);
   }
   return (int)hipGetLastError();

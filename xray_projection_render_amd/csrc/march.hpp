@@ -72,6 +72,9 @@ struct MarchParams {
   // radius r settles the next floor(r * skip_inv) samples without
   // evaluating them.  0 disables.
   float skip_inv;
+  // Guarded simple march, per-scene kernel: samples per node visit (2 or 4).
+  int32_t block;
+  int32_t pad4_;
 };
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with

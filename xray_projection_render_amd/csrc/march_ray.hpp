@@ -20,8 +20,11 @@
 
 namespace xrp {
 
-// Samples per node visit of the guarded FP32 simple march.
+// Samples per node visit of the guarded FP32 simple march (the per-scene
+// kernel picks 2 or 4 by scene size, the batched kernel uses 4), and of the
+// hierarchical march's coarse and fine blocks.
 constexpr int kBlock = 4;
+constexpr int kHierBlock = 4;
 
 // Counters one ray adds to the launch's stats.
 struct RayCounts {
@@ -30,7 +33,7 @@ struct RayCounts {
 
 // Optical depth T (flat field included) of pixel (i, j) of view v (v counts
 // from p.view_offset), simple integrator.
-template <int PREC>
+template <int PREC, int B>
 __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, int i, int j,
                                                  RayCounts& rc) {
   const DevView& V = p.views[p.view_offset + v];
@@ -66,7 +69,7 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
         } else {
           const float ds32 = (float)ds;
           const float d0 = (float)r.d[0], d1 = (float)r.d[1], d2 = (float)r.d[2];
-          // One flat loop, one block of kBlock samples per iteration, so a
+          // One flat loop, one block of B samples per iteration, so a
           // wave's lanes stay in step whatever each one skips.  Positions
           // are FP32 strides of at most kChunk - 1 steps from an FP64
           // anchor at sample ka, s_ka = smin + ka ds (within ~K ulp of the
@@ -74,7 +77,7 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
           int k = k_lo, ka = -kChunk;
           float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
           while (k <= k_hi) {
-            if (k + (kBlock - 1) - ka >= kChunk) {
+            if (k + (B - 1) - ka >= kChunk) {
               ka = k;
               const double sa = p.smin + (double)ka * ds;
               a0 = (float)(r.o[0] + r.d[0] * sa);
@@ -82,31 +85,31 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
               a2 = (float)(r.o[2] + r.d[2] * sa);
             }
             const int j0 = (int)(k - ka);
-            PBlock<kBlock> P;
+            PBlock<B> P;
 #pragma unroll
-            for (int q = 0; q < kBlock; ++q) {
+            for (int q = 0; q < B; ++q) {
               float sj = (float)(j0 + q) * ds32;
               P.x[q] = fmaf(d0, sj, a0);
               P.y[q] = fmaf(d1, sj, a1);
               P.z[q] = fmaf(d2, sj, a2);
             }
-            deform_blk<kBlock>(p.defs, p.n_defs, P);
+            deform_blk<B>(p.defs, p.n_defs, P);
             const int left = k_hi - k + 1;
-            const unsigned live = left >= kBlock ? (1u << kBlock) - 1u : (1u << left) - 1u;
-            double v[kBlock];
+            const unsigned live = left >= B ? (1u << B) - 1u : (1u << left) - 1u;
+            double v[B];
             unsigned amb = 0;
             float dist = INFINITY;
-            root_blk<kBlock, true>(p.nodes, P, live, v, amb, dist);
+            root_blk<B, true>(p.nodes, P, live, v, amb, dist);
             // FP64 re-evaluation of the ambiguous samples (all of them under
             // audit) through one call site, then T in sample order.
-            const bool last_amb = (amb >> (kBlock - 1)) & 1u;
+            const bool last_amb = (amb >> (B - 1)) & 1u;
             unsigned redo = p.audit ? live : amb;
             while (redo) {
               const int q = __builtin_ctz(redo);
               redo &= redo - 1u;
               double vq = v[0];
 #pragma unroll
-              for (int t = 1; t < kBlock; ++t)
+              for (int t = 1; t < B; ++t)
                 if (q == t) vq = v[t];
               const double s = p.s_table[k + q];
               const double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
@@ -116,23 +119,23 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
               else if (e != vq)
                 ++n_miss;
 #pragma unroll
-              for (int t = 0; t < kBlock; ++t)
+              for (int t = 0; t < B; ++t)
                 if (q == t) v[t] = e;
             }
 #pragma unroll
-            for (int q = 0; q < kBlock; ++q)
+            for (int q = 0; q < B; ++q)
               if ((live >> q) & 1u) T += (v[q] * p.dm) * ds;
             const int n_live = __builtin_popcount(live);
             n_samples += (unsigned)n_live;
             k += n_live;
             // Safe radius of the block's last sample: within dist of its
-            // object-space position the exact density is v[kBlock-1], and
+            // object-space position the exact density is v[B-1], and
             // sample k+j lies at most lipschitz * (j+1) * ds from it.
-            if (live == (1u << kBlock) - 1u && !last_amb) {
+            if (live == (1u << B) - 1u && !last_amb) {
               const float f = dist * p.skip_inv;
               if (f >= 1.0f && k <= k_hi) {
                 const int m = min(f < 65536.0f ? (int)f : 65536, k_hi - k + 1);
-                const double rho_m = v[kBlock - 1];
+                const double rho_m = v[B - 1];
                 if (p.audit) {
                   for (int j = 0; j < m; ++j) {
                     double s = p.s_table[k + j];
@@ -252,7 +255,7 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
 template <int PREC>
 __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int i, int j,
                                                bool valid, RayCounts& rc) {
-  constexpr int B = PREC == PREC_FP64 ? 1 : kBlock;
+  constexpr int B = PREC == PREC_FP64 ? 1 : kHierBlock;
   __shared__ HierQueue queues[kHierWaves];
   HierQueue& Q = queues[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
@@ -412,22 +415,23 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
 
 // T of pixel (i, j) of view v.  Every lane of the wave calls it together;
 // lanes with valid = false have no pixel and their return value is unused.
-template <int PREC, int INTEG>
+// B: samples per node visit of the guarded simple march.
+template <int PREC, int INTEG, int B = kBlock>
 __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
   if constexpr (INTEG == INT_HIERARCHICAL) {
     return march_T_hier<PREC>(p, v, i, j, valid, rc);
   } else {
-    return valid ? march_T_simple<PREC>(p, v, i, j, rc) : 0.0;
+    return valid ? march_T_simple<PREC, B>(p, v, i, j, rc) : 0.0;
   }
 }
 
 // March pixel (i, j) of view v and store its transmittance (the image is
 // written at out + v * out_view_stride).  Called by every lane of the wave.
-template <int PREC, int INTEG>
+template <int PREC, int INTEG, int B = kBlock>
 __device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
-  const double T = march_T<PREC, INTEG>(p, v, i, j, valid, rc);
+  const double T = march_T<PREC, INTEG, B>(p, v, i, j, valid, rc);
   if (valid) store_pixel(p, v, i, j, T);
 }
 
