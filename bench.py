@@ -451,6 +451,12 @@ def main() -> int:
             block = 4 if scene.layout()[0] >= 24 else 2
             roof["kernel"] = f"march_kernel<PREC_GUARDED,INT_SIMPLE,{block}>"
     roof["skipped_samples"] = st.get("skipped_samples", 0)
+    if roof["bound"] == "hbm":
+        # the gathers mostly hit L2: also state them against the measured L2
+        # read bandwidth (xrp_diag_l2_bandwidth)
+        l2 = X.diag_l2_bandwidth(ctx)
+        roof["l2_peak_measured"] = l2
+        roof["l2_frac"] = achieved / l2 if l2 > 0 else None
     roof["kernel_ms"] = k_ms
     roof["per_sample"] = (f"{BYTES_PER_SAMPLE[args.workload]:.0f} B" if args.workload
                           in BYTES_PER_SAMPLE else f"{FLOPS_PER_SAMPLE[args.workload]:.0f} flop")

@@ -346,6 +346,11 @@ void xrp_surrogate_destroy(xrp_surrogate *s);
 /* Diagnostic: D = A B for 16x16 row-major float matrices through one
  * v_mfma_f32_16x16x16_bf16 with the operand layout the surrogate uses. */
 int xrp_diag_mfma_bf16(xrp_context *ctx, const float *a, const float *b, float *d);
+/* Diagnostic, no counterpart in the reference: measured L2 read bandwidth
+ * (GB/s, best of 3 timed runs) of a kernel that streams one 2 MiB region
+ * per XCD through its L2 with 16-byte loads; the second ceiling the voxel
+ * kernel's gathers are compared with. */
+int xrp_diag_l2_bandwidth(xrp_context *ctx, double *gbps);
 
 /* ---- reference JSON API (api.go:82-180) ----------------------------------- */
 /* Renders as the reference's RenderProjections does (PNG frames,

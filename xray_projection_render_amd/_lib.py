@@ -201,6 +201,7 @@ def lib() -> ctypes.CDLL:
         "xrp_surrogate_gradients": (I, [P, FP, FP, ctypes.c_int64, FP, DP]),
         "xrp_surrogate_destroy": (None, [P]),
         "xrp_diag_mfma_bf16": (I, [P, FP, FP, FP]),
+        "xrp_diag_l2_bandwidth": (I, [P, DP]),
         "RenderProjections": (ctypes.POINTER(ctypes.c_char), [ctypes.c_char_p]),
         "FreeString": (None, [ctypes.POINTER(ctypes.c_char)]),
     }
@@ -616,6 +617,13 @@ class Surrogate:
             self.close()
         except Exception:
             pass
+
+
+def diag_l2_bandwidth(ctx: Context) -> float:
+    """Measured L2 read bandwidth of the device, GB/s (xrp_diag_l2_bandwidth)."""
+    v = ctypes.c_double()
+    check(lib().xrp_diag_l2_bandwidth(ctx.ptr, ctypes.byref(v)))
+    return float(v.value)
 
 
 def diag_mfma_bf16(ctx: Context, a: np.ndarray, b: np.ndarray) -> np.ndarray:

@@ -1801,4 +1801,38 @@ int xrp_diag_mfma_bf16(xrp_context* ctx, const float* a, const float* b, float* 
   return XRP_OK;
 }
 
+int xrp_diag_l2_bandwidth(xrp_context* ctx, double* gbps) {
+  if (!ctx || !gbps) return fail(XRP_ERR_NULL, "NULL argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+  const int workgroups = 8 * std::max(1, prop.multiProcessorCount);
+  const int iters = 4096;
+  const size_t bytes = xrp::l2_probe_buffer_bytes();
+  DevBuf buf, out;
+  HIPCHK(buf.ensure(bytes));
+  HIPCHK(out.ensure((size_t)workgroups * 256 * sizeof(unsigned)));
+  hipError_t he = hipMemsetAsync(buf.p, 1, bytes, ctx->stream);
+  float best = 0.0f;
+  for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {
+    he = hipEventRecord(ctx->ev0, ctx->stream);
+    int e = he == hipSuccess ? xrp::launch_l2_probe(buf.p, workgroups, iters,
+                                                    (unsigned*)out.p, ctx->stream)
+                             : 0;
+    if (e) he = (hipError_t)e;
+    if (he == hipSuccess) he = hipEventRecord(ctx->ev1, ctx->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(ctx->ev1);
+    float ms = 0.0f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (he == hipSuccess && rep > 0 && ms > 0.0f && (best == 0.0f || ms < best)) best = ms;
+  }
+  buf.release();
+  out.release();
+  if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+  const double read = (double)workgroups * 256.0 * iters * 16.0;
+  *gbps = best > 0.0f ? read / (best * 1e-3) / 1e9 : 0.0;
+  return XRP_OK;
+}
+
 }  // extern "C"

@@ -91,6 +91,11 @@ inline bool voxel_layout_linear(int layout) {
   return layout == VOX_LINEAR || layout == VOX_LINEAR_CACHED;
 }
 int launch_voxel(const MarchParams& p, int layout, void* stream);
+// L2 read-bandwidth probe: workgroups x 256 lanes each read iters 16-byte
+// words from a buffer of l2_probe_buffer_bytes(); out gets workgroups x 256
+// words.
+size_t l2_probe_buffer_bytes();
+int launch_l2_probe(const void* buf, int workgroups, int iters, unsigned* out, void* stream);
 // Exact march of a list: with vals, items are (view, i, j) pixels and
 // vals[k] receives the transmittance; without, items are (view, ti, tj)
 // 16x16 tiles rendered into p.out.

@@ -486,6 +486,42 @@ __global__ __launch_bounds__(64) void voxel_lds_kernel(MarchParams p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// L2 read-bandwidth probe, the second ceiling the voxel kernel is compared
+// with.  Workgroup b reads 4 KiB chunks of region b % 8 (one 2 MiB region per
+// XCD, workgroups being dealt to the 8 XCDs in turn), moving to another chunk
+// of the region every iteration: 512 chunks per region defeat the 32 KiB L1,
+// and the region fits the XCD's 4 MiB L2.  Each lane folds what it reads into
+// one word so the loads stay live.
+// ---------------------------------------------------------------------------
+constexpr int kL2Regions = 8;
+constexpr int kL2ChunkVec = 256;                         // uint4 per chunk (4 KiB)
+constexpr int kL2ChunksPerRegion = 512;                  // 2 MiB per region
+__global__ __launch_bounds__(256) void l2_probe_kernel(const uint4* __restrict__ buf,
+                                                       int iters, unsigned* __restrict__ out) {
+  const int region = blockIdx.x % kL2Regions;
+  const int slot = blockIdx.x / kL2Regions;
+  const uint4* base = buf + (size_t)region * kL2ChunksPerRegion * kL2ChunkVec + threadIdx.x;
+  unsigned acc = 0;
+  for (int it = 0; it < iters; ++it) {
+    const int chunk = (slot + it * 7) & (kL2ChunksPerRegion - 1);
+    const uint4 v = base[(size_t)chunk * kL2ChunkVec];
+    acc ^= v.x + v.y + v.z + v.w;
+  }
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+size_t l2_probe_buffer_bytes() {
+  return (size_t)kL2Regions * kL2ChunksPerRegion * kL2ChunkVec * sizeof(uint4);
+}
+
+int launch_l2_probe(const void* buf, int workgroups, int iters, unsigned* out, void* stream) {
+  if (workgroups <= 0 || iters <= 0) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(l2_probe_kernel, dim3(workgroups), dim3(256), 0, (hipStream_t)stream,
+                     (const uint4*)buf, iters, out);
+  return (int)hipGetLastError();
+}
+
 int launch_voxel(const MarchParams& p, int layout, void* stream) {
   if (layout == VOX_LDS) {
     int tiles = (p.res + 7) / 8;
