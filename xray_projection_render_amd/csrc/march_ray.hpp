@@ -8,7 +8,9 @@
 //  * PREC_FP64 evaluates the density tree in FP64 exactly as the oracle.
 //  * PREC_GUARDED decides primitive membership in FP32 with guard bands and
 //    re-evaluates in FP64 any sample inside a band, so the densities summed
-//    are the oracle's.
+//    are the oracle's.  Samples inside an evaluated sample's safe radius
+//    (eval_block.hpp) are settled without evaluation: their exact density is
+//    that sample's, and their terms are added to T in order all the same.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -384,11 +386,34 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
                 left += dsf;
               }
             }
-            eval_along<PREC, B>(p, o, d, s, live, rf, n_reeval, n_miss);
+            constexpr bool kDist = PREC == PREC_GUARDED;
+            float dist = INFINITY;
+            bool last_amb = true;
+            eval_along<PREC, B, kDist>(p, o, d, s, live, rf, n_reeval, n_miss, &dist,
+                                       &last_amb);
 #pragma unroll
             for (int q = 0; q < B; ++q)
               if ((live >> q) & 1u) S += (rf[q] * dm) * dsf;
             n_fine += (unsigned long long)__builtin_popcount(live);
+            // Fine samples inside the last one's safe radius: same terms,
+            // same left += ds walk, no evaluation.
+            if constexpr (kDist) {
+              float f = dist * p.skip_inv * 10.0f;  // skip_inv is per DS = 10 ds
+              if (live == (1u << B) - 1u && !last_amb && f >= 1.0f) {
+                const double c = (rf[B - 1] * dm) * dsf;
+                for (int m = f < 64.0f ? (int)f : 64; m > 0 && left < right; --m) {
+                  if (p.audit) {
+                    const double e2 = density64(p, o[0] + d[0] * left, o[1] + d[1] * left,
+                                                o[2] + d[2] * left);
+                    if (e2 != rf[B - 1]) ++n_miss;
+                  }
+                  S += c;
+                  left += dsf;
+                  ++n_fine;
+                  ++n_skip;
+                }
+              }
+            }
           }
           Q.sum[e] = S;
         }
