@@ -33,130 +33,146 @@ struct RayCounts {
   unsigned long long samples = 0, reeval = 0, miss = 0, fine = 0, skipped = 0;
 };
 
+// First and last sample index of a ray's clipped segment (main.go:144-154;
+// s_k = s_table[k], s_{k+1} = s_k + ds).  Empty when k_hi < k_lo.
+__device__ __forceinline__ void clip_range(const MarchParams& p, double t_in, double t_out,
+                                           int& k_lo, int& k_hi) {
+  k_lo = 0;
+  k_hi = p.K - 1;
+  if (p.clip) {
+    double a = floor((t_in - p.smin) / p.ds) - 2.0;
+    double b = ceil((t_out - p.smin) / p.ds) + 2.0;
+    k_lo = a > 0.0 ? (a < (double)p.K ? (int)a : p.K) : 0;
+    k_hi = b < (double)(p.K - 1) ? (b > -1.0 ? (int)b : -1) : p.K - 1;
+  }
+}
+
 // Optical depth T (flat field included) of pixel (i, j) of view v (v counts
-// from p.view_offset), simple integrator.
-template <int PREC, int B>
-__device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, int i, int j,
-                                                 RayCounts& rc) {
-  const DevView& V = p.views[p.view_offset + v];
+// from p.view_offset), simple integrator, FP64 evaluator.
+__device__ __forceinline__ double march_T_simple64(const MarchParams& p, int v, int i, int j,
+                                                   RayCounts& rc) {
+  double T = p.flat_field;
+  Ray r = make_ray(p, p.views[p.view_offset + v], i, j);
+  double t_in, t_out;
+  if (clip_ray(p, r, t_in, t_out)) {
+    int k_lo, k_hi;
+    clip_range(p, t_in, t_out, k_lo, k_hi);
+    if (k_lo <= k_hi) {
+      const double ds = p.ds;
+      double s = p.s_table[k_lo];
+      for (int k = k_lo; k <= k_hi; ++k) {
+        double x = r.o[0] + r.d[0] * s;
+        double y = r.o[1] + r.d[1] * s;
+        double z = r.o[2] + r.d[2] * s;
+        T += (density64(p, x, y, z) * p.dm) * ds;
+        s += ds;
+      }
+      rc.samples += (unsigned long long)(k_hi - k_lo + 1);
+    }
+  }
+  return T;
+}
+
+// Optical depth T of pixel (i, j) of view v, simple integrator, guarded FP32
+// membership with B samples per node visit and inline FP64 re-evaluation of
+// the samples inside a guard band.  Those terms are summed, in sample order,
+// into Ta, the others into T, and the result is T + Ta: the same two sums,
+// and so the same bits, as the queued form below.
+template <int B>
+__device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, int v, int i,
+                                                         int j, RayCounts& rc) {
   // per-ray counts fit 32 bits (a ray has fewer than 2^31 samples)
   unsigned n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
-  double T = p.flat_field;
-  {
-    Ray r = make_ray(p, V, i, j);
-    double t_in, t_out;
-    if (clip_ray(p, r, t_in, t_out)) {
-      {
-        // main.go:144-154; s_k = s_table[k], s_{k+1} = s_k + ds.
-        int k_lo = 0, k_hi = p.K - 1;
-        if (p.clip) {
-          double a = floor((t_in - p.smin) / p.ds) - 2.0;
-          double b = ceil((t_out - p.smin) / p.ds) + 2.0;
-          k_lo = a > 0.0 ? (a < (double)p.K ? (int)a : p.K) : 0;
-          k_hi = b < (double)(p.K - 1) ? (b > -1.0 ? (int)b : -1) : p.K - 1;
+  const double ds = p.ds, dm = p.dm;
+  double T = p.flat_field, Ta = 0.0;
+  Ray r = make_ray(p, p.views[p.view_offset + v], i, j);
+  double t_in, t_out;
+  if (clip_ray(p, r, t_in, t_out)) {
+    int k, k_hi;
+    clip_range(p, t_in, t_out, k, k_hi);
+    const float ds32 = (float)ds;
+    const float d0 = (float)r.d[0], d1 = (float)r.d[1], d2 = (float)r.d[2];
+    // One block of B samples per iteration.  Positions are FP32 strides of
+    // at most kChunk - 1 steps from an FP64 anchor at sample ka, s_ka =
+    // smin + ka ds (within ~K ulp of the accumulated table value; the error
+    // model allows for it).
+    int ka = -kChunk;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    while (k <= k_hi) {
+      if (k + (B - 1) - ka >= kChunk) {
+        ka = k;
+        const double sa = p.smin + (double)ka * ds;
+        a0 = (float)(r.o[0] + r.d[0] * sa);
+        a1 = (float)(r.o[1] + r.d[1] * sa);
+        a2 = (float)(r.o[2] + r.d[2] * sa);
+      }
+      const int j0 = k - ka;
+      PBlock<B> P;
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        float sj = (float)(j0 + q) * ds32;
+        P.x[q] = fmaf(d0, sj, a0);
+        P.y[q] = fmaf(d1, sj, a1);
+        P.z[q] = fmaf(d2, sj, a2);
+      }
+      deform_blk<B>(p.defs, p.n_defs, P);
+      const int left = k_hi - k + 1;
+      const unsigned live = left >= B ? (1u << B) - 1u : (1u << left) - 1u;
+      double vv[B];
+      unsigned amb = 0;
+      float dist = INFINITY;
+      root_blk<B, true>(p.nodes, P, live, vv, amb, dist);
+      const bool last_amb = (amb >> (B - 1)) & 1u;
+      // FP64 re-evaluation (under audit of every sample; the FP32-clear
+      // ones must agree) through one call site
+      unsigned redo = p.audit ? live : amb;
+      while (redo) {
+        const int q = __builtin_ctz(redo);
+        redo &= redo - 1u;
+        double vq = vv[0];
+#pragma unroll
+        for (int t = 1; t < B; ++t)
+          if (q == t) vq = vv[t];
+        const double s = p.s_table[k + q];
+        const double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                   r.o[2] + r.d[2] * s);
+        if ((amb >> q) & 1u) {
+          ++n_reeval;
+          Ta += (e * dm) * ds;
+        } else if (e != vq) {
+          ++n_miss;
         }
-        const double ds = p.ds;
-        if constexpr (PREC == PREC_FP64) {
-          if (k_lo <= k_hi) {
-            double s = p.s_table[k_lo];
-            for (int k = k_lo; k <= k_hi; ++k) {
-              double x = r.o[0] + r.d[0] * s;
-              double y = r.o[1] + r.d[1] * s;
-              double z = r.o[2] + r.d[2] * s;
-              T += (density64(p, x, y, z) * p.dm) * ds;
-              s += ds;
-            }
-            n_samples += (unsigned)(k_hi - k_lo + 1);
-          }
-        } else {
-          const float ds32 = (float)ds;
-          const float d0 = (float)r.d[0], d1 = (float)r.d[1], d2 = (float)r.d[2];
-          // One flat loop, one block of B samples per iteration, so a
-          // wave's lanes stay in step whatever each one skips.  Positions
-          // are FP32 strides of at most kChunk - 1 steps from an FP64
-          // anchor at sample ka, s_ka = smin + ka ds (within ~K ulp of the
-          // accumulated table value; the error model allows for it).
-          int k = k_lo, ka = -kChunk;
-          float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-          while (k <= k_hi) {
-            if (k + (B - 1) - ka >= kChunk) {
-              ka = k;
-              const double sa = p.smin + (double)ka * ds;
-              a0 = (float)(r.o[0] + r.d[0] * sa);
-              a1 = (float)(r.o[1] + r.d[1] * sa);
-              a2 = (float)(r.o[2] + r.d[2] * sa);
-            }
-            const int j0 = (int)(k - ka);
-            PBlock<B> P;
+      }
 #pragma unroll
-            for (int q = 0; q < B; ++q) {
-              float sj = (float)(j0 + q) * ds32;
-              P.x[q] = fmaf(d0, sj, a0);
-              P.y[q] = fmaf(d1, sj, a1);
-              P.z[q] = fmaf(d2, sj, a2);
-            }
-            deform_blk<B>(p.defs, p.n_defs, P);
-            const int left = k_hi - k + 1;
-            const unsigned live = left >= B ? (1u << B) - 1u : (1u << left) - 1u;
-            double v[B];
-            unsigned amb = 0;
-            float dist = INFINITY;
-            root_blk<B, true>(p.nodes, P, live, v, amb, dist);
-            // FP64 re-evaluation of the ambiguous samples (all of them under
-            // audit) through one call site, then T in sample order.
-            const bool last_amb = (amb >> (B - 1)) & 1u;
-            unsigned redo = p.audit ? live : amb;
-            while (redo) {
-              const int q = __builtin_ctz(redo);
-              redo &= redo - 1u;
-              double vq = v[0];
-#pragma unroll
-              for (int t = 1; t < B; ++t)
-                if (q == t) vq = v[t];
-              const double s = p.s_table[k + q];
-              const double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                         r.o[2] + r.d[2] * s);
-              if ((amb >> q) & 1u)
-                ++n_reeval;
-              else if (e != vq)
-                ++n_miss;
-#pragma unroll
-              for (int t = 0; t < B; ++t)
-                if (q == t) v[t] = e;
-            }
-#pragma unroll
-            for (int q = 0; q < B; ++q)
-              if ((live >> q) & 1u) T += (v[q] * p.dm) * ds;
-            const int n_live = __builtin_popcount(live);
-            n_samples += (unsigned)n_live;
-            k += n_live;
-            // Safe radius of the block's last sample: within dist of its
-            // object-space position the exact density is v[B-1], and
-            // sample k+j lies at most lipschitz * (j+1) * ds from it.
-            if (live == (1u << B) - 1u && !last_amb) {
-              const float f = dist * p.skip_inv;
-              if (f >= 1.0f && k <= k_hi) {
-                const int m = min(f < 65536.0f ? (int)f : 65536, k_hi - k + 1);
-                const double rho_m = v[B - 1];
-                if (p.audit) {
-                  for (int j = 0; j < m; ++j) {
-                    double s = p.s_table[k + j];
-                    double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                         r.o[2] + r.d[2] * s);
-                    if (e != rho_m) ++n_miss;
-                  }
-                }
-                // the terms the evaluated samples would add, in order (a
-                // zero term leaves T as it is)
-                const double c = (rho_m * p.dm) * ds;
-                if (c != 0.0)
-                  for (int j = 0; j < m; ++j) T += c;
-                n_samples += (unsigned)m;
-                n_skip += (unsigned)m;
-                k += m;
-              }
+      for (int q = 0; q < B; ++q)
+        if (((live & ~amb) >> q) & 1u) T += (vv[q] * dm) * ds;
+      const int n_live = __builtin_popcount(live);
+      n_samples += (unsigned)n_live;
+      k += n_live;
+      // Safe radius of the block's last sample: within dist of its
+      // object-space position the exact density is vv[B-1], and sample
+      // k+j lies at most lipschitz * (j+1) * ds from it.
+      if (live == (1u << B) - 1u && !last_amb) {
+        const float f = dist * p.skip_inv;
+        if (f >= 1.0f && k <= k_hi) {
+          const int m = min(f < 65536.0f ? (int)f : 65536, k_hi - k + 1);
+          const double rho_m = vv[B - 1];
+          if (p.audit) {
+            for (int jj = 0; jj < m; ++jj) {
+              double s = p.s_table[k + jj];
+              double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                   r.o[2] + r.d[2] * s);
+              if (e != rho_m) ++n_miss;
             }
           }
+          // the terms the evaluated samples would add, in order (a zero
+          // term leaves T as it is)
+          const double c = (rho_m * dm) * ds;
+          if (c != 0.0)
+            for (int jj = 0; jj < m; ++jj) T += c;
+          n_samples += (unsigned)m;
+          n_skip += (unsigned)m;
+          k += m;
         }
       }
     }
@@ -165,7 +181,206 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
   rc.reeval += n_reeval;
   rc.miss += n_miss;
   rc.skipped += n_skip;
-  return T;
+  return T + Ta;
+}
+
+// Waves per workgroup of the kernels whose march keeps a per-wave queue in
+// LDS (every such kernel is bounded to 256 lanes).
+constexpr int kQueueWaves = 4;
+
+// Per-wave queue of ambiguous samples awaiting FP64 re-evaluation.
+constexpr int kAmbCap = 512;
+struct AmbQueue {
+  double val[kAmbCap];     // FP64 density of the sample (dm not applied)
+  int32_t k[kAmbCap];      // its sample index
+  int16_t link[kAmbCap];   // the owning lane's next entry, -1 for none
+  uint8_t owner[kAmbCap];  // lane whose ray it belongs to
+};
+
+// Optical depth T of pixel (i, j) of view v, simple integrator, guarded FP32
+// membership with B samples per node visit.  Every lane of the wave calls it
+// together (lanes without a pixel pass valid = false and still help with the
+// FP64 re-evaluations); T is meaningful for valid lanes.
+//
+// Queued form of march_T_guarded_inline, for kernels with registers to
+// spare (the drain's FP64 evaluator and the queue state spill at 4 waves per
+// SIMD).  A sample inside a guard band waits in a per-wave queue in LDS;
+// once the queue is half full, or every lane has finished, the wave
+// re-evaluates the queued samples 64 at a time, one per lane, instead of
+// stalling 63 lanes behind each one.  Each lane adds those terms in sample
+// order into Ta, so the result has the same bits as the inline form.  Under
+// audit the re-evaluations are made inline.
+template <int B>
+__device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, int i, int j,
+                                                  bool valid, RayCounts& rc) {
+  __shared__ AmbQueue queues[kQueueWaves];
+  AmbQueue& Q = queues[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  // per-ray counts fit 32 bits (a ray has fewer than 2^31 samples)
+  unsigned n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
+  const double ds = p.ds, dm = p.dm;
+  double T = p.flat_field;  // terms decided in FP32 or settled, in sample order
+  double Ta = 0.0;          // FP64 re-evaluated terms, in sample order
+  Ray r;
+  r.o[0] = r.o[1] = r.o[2] = 0.0;
+  r.d[0] = r.d[1] = 0.0;
+  r.d[2] = 1.0;
+  int k = 0, k_hi = -1;
+  if (valid) {
+    r = make_ray(p, p.views[p.view_offset + v], i, j);
+    double t_in, t_out;
+    if (clip_ray(p, r, t_in, t_out)) clip_range(p, t_in, t_out, k, k_hi);
+  }
+  const float ds32 = (float)ds;
+  const float d0 = (float)r.d[0], d1 = (float)r.d[1], d2 = (float)r.d[2];
+  // One block of B samples per iteration.  Positions are FP32 strides of at
+  // most kChunk - 1 steps from an FP64 anchor at sample ka, s_ka = smin +
+  // ka ds (within ~K ulp of the accumulated table value; the error model
+  // allows for it).
+  int ka = -kChunk;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  int head = -1, tail = -1;  // this lane's queue entries, in sample order
+  int queued = 0;            // entries in the wave's queue (same on every lane)
+  for (;;) {
+    const int kb = k;
+    unsigned defer = 0;  // this block's samples to re-evaluate later
+    if (k <= k_hi) {
+      if (k + (B - 1) - ka >= kChunk) {
+        ka = k;
+        const double sa = p.smin + (double)ka * ds;
+        a0 = (float)(r.o[0] + r.d[0] * sa);
+        a1 = (float)(r.o[1] + r.d[1] * sa);
+        a2 = (float)(r.o[2] + r.d[2] * sa);
+      }
+      const int j0 = k - ka;
+      PBlock<B> P;
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        float sj = (float)(j0 + q) * ds32;
+        P.x[q] = fmaf(d0, sj, a0);
+        P.y[q] = fmaf(d1, sj, a1);
+        P.z[q] = fmaf(d2, sj, a2);
+      }
+      deform_blk<B>(p.defs, p.n_defs, P);
+      const int left = k_hi - k + 1;
+      const unsigned live = left >= B ? (1u << B) - 1u : (1u << left) - 1u;
+      double vv[B];
+      unsigned amb = 0;
+      float dist = INFINITY;
+      root_blk<B, true>(p.nodes, P, live, vv, amb, dist);
+      n_reeval += (unsigned)__builtin_popcount(amb);
+      if (p.audit) {
+        // FP64 inline: the ambiguous samples, or under audit every sample
+        // (the FP32-clear ones must agree)
+        unsigned redo = p.audit ? live : amb;
+        while (redo) {
+          const int q = __builtin_ctz(redo);
+          redo &= redo - 1u;
+          double vq = vv[0];
+#pragma unroll
+          for (int t = 1; t < B; ++t)
+            if (q == t) vq = vv[t];
+          const double s = p.s_table[k + q];
+          const double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                     r.o[2] + r.d[2] * s);
+          if ((amb >> q) & 1u)
+            Ta += (e * dm) * ds;
+          else if (e != vq)
+            ++n_miss;
+        }
+      } else {
+        defer = amb;
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q)
+        if (((live & ~amb) >> q) & 1u) T += (vv[q] * dm) * ds;
+      const int n_live = __builtin_popcount(live);
+      n_samples += (unsigned)n_live;
+      k += n_live;
+      // Safe radius of the block's last sample: within dist of its
+      // object-space position the exact density is vv[B-1], and sample
+      // k+j lies at most lipschitz * (j+1) * ds from it.
+      if (live == (1u << B) - 1u && !((amb >> (B - 1)) & 1u)) {
+        const float f = dist * p.skip_inv;
+        if (f >= 1.0f && k <= k_hi) {
+          const int m = min(f < 65536.0f ? (int)f : 65536, k_hi - k + 1);
+          const double rho_m = vv[B - 1];
+          if (p.audit) {
+            for (int jj = 0; jj < m; ++jj) {
+              double s = p.s_table[k + jj];
+              double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
+                                   r.o[2] + r.d[2] * s);
+              if (e != rho_m) ++n_miss;
+            }
+          }
+          // the terms the evaluated samples would add, in order (a zero
+          // term leaves T as it is)
+          const double c = (rho_m * dm) * ds;
+          if (c != 0.0)
+            for (int jj = 0; jj < m; ++jj) T += c;
+          n_samples += (unsigned)m;
+          n_skip += (unsigned)m;
+          k += m;
+        }
+      }
+    }
+    // Queue this block's deferred samples in (sample, lane) order.
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const bool t = (defer >> q) & 1u;
+      const unsigned long long mask = __ballot(t);
+      if (t) {
+        const int pos = queued + (int)__popcll(mask & __lanemask_lt());
+        Q.k[pos] = kb + q;
+        Q.owner[pos] = (uint8_t)lane;
+        Q.link[pos] = -1;
+        if (tail >= 0)
+          Q.link[tail] = (int16_t)pos;
+        else
+          head = pos;
+        tail = pos;
+      }
+      queued += (int)__popcll(mask);
+    }
+    const bool more = __ballot(k <= k_hi) != 0ull;
+    if (queued > kAmbCap - 64 * B || (!more && queued > 0)) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int base = 0; base < queued; base += 64) {
+        const int e = base + lane;
+        const bool has = e < queued;
+        const int src = has ? (int)Q.owner[e] : lane;
+        const int kk = has ? Q.k[e] : 0;
+        double o[3], d[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          o[c] = __shfl(r.o[c], src, 64);
+          d[c] = __shfl(r.d[c], src, 64);
+        }
+        if (has) {
+          const double s = p.s_table[kk];
+          Q.val[e] = density64(p, o[0] + d[0] * s, o[1] + d[1] * s, o[2] + d[2] * s);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      while (head >= 0) {
+        Ta += (Q.val[head] * dm) * ds;
+        head = Q.link[head];
+      }
+      tail = -1;
+      queued = 0;
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!more) break;
+  }
+  rc.samples += n_samples;
+  rc.reeval += n_reeval;
+  rc.miss += n_miss;
+  rc.skipped += n_skip;
+  return T + Ta;
 }
 
 // ---------------------------------------------------------------------------
@@ -184,7 +399,6 @@ __device__ __forceinline__ double march_T_simple(const MarchParams& p, int v, in
 // from the reference's only in the FP64 summation order.
 
 constexpr int kHierCap = 512;   // queued toggles per wave
-constexpr int kHierWaves = 4;   // waves per workgroup: kernels using it are bounded to 256 lanes
 
 struct HierQueue {
   double sum[kHierCap];     // fine sum of the step: sum of (rho * dm) * DS/10
@@ -258,7 +472,7 @@ template <int PREC>
 __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int i, int j,
                                                bool valid, RayCounts& rc) {
   constexpr int B = PREC == PREC_FP64 ? 1 : kHierBlock;
-  __shared__ HierQueue queues[kHierWaves];
+  __shared__ HierQueue queues[kQueueWaves];
   HierQueue& Q = queues[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const double DS = p.ds, dsf = p.ds_fine, dm = p.dm;
@@ -440,23 +654,28 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
 
 // T of pixel (i, j) of view v.  Every lane of the wave calls it together;
 // lanes with valid = false have no pixel and their return value is unused.
-// B: samples per node visit of the guarded simple march.
-template <int PREC, int INTEG, int B = kBlock>
+// B: samples per node visit of the guarded simple march; DEFER: queue its
+// FP64 re-evaluations per wave (same result).
+template <int PREC, int INTEG, int B = kBlock, bool DEFER = false>
 __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
   if constexpr (INTEG == INT_HIERARCHICAL) {
     return march_T_hier<PREC>(p, v, i, j, valid, rc);
+  } else if constexpr (PREC == PREC_FP64) {
+    return valid ? march_T_simple64(p, v, i, j, rc) : 0.0;
+  } else if constexpr (DEFER) {
+    return march_T_guarded<B>(p, v, i, j, valid, rc);
   } else {
-    return valid ? march_T_simple<PREC, B>(p, v, i, j, rc) : 0.0;
+    return valid ? march_T_guarded_inline<B>(p, v, i, j, rc) : 0.0;
   }
 }
 
 // March pixel (i, j) of view v and store its transmittance (the image is
 // written at out + v * out_view_stride).  Called by every lane of the wave.
-template <int PREC, int INTEG, int B = kBlock>
+template <int PREC, int INTEG, int B = kBlock, bool DEFER = false>
 __device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
-  const double T = march_T<PREC, INTEG, B>(p, v, i, j, valid, rc);
+  const double T = march_T<PREC, INTEG, B, DEFER>(p, v, i, j, valid, rc);
   if (valid) store_pixel(p, v, i, j, T);
 }
 
