@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchGroup g,
       for (int bj = 0; bj < g.wave_cols; ++bj) {
         const int i = ti * tile_i + bi * 8 + (lane >> 3);
         const int j = tj * tile_j + bj * 8 + (lane & 7);
-        march_ray<PREC, INTEG, kBlock, true>(p, v, i, j, i < p.res && j < p.res, rc);
+        march_ray<PREC, INTEG, 4, kBlock, true>(p, v, i, j, i < p.res && j < p.res, rc);
       }
     if (p.stats) {
       wave_add(p.stats + ST_SAMPLES, wave_sum(rc.samples));

@@ -184,9 +184,9 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
   return T + Ta;
 }
 
-// Waves per workgroup of the kernels whose march keeps a per-wave queue in
-// LDS (every such kernel is bounded to 256 lanes).
-constexpr int kQueueWaves = 4;
+// The marches that keep a per-wave queue in LDS take the number of waves per
+// workgroup, QW, as a template argument (the kernel's launch bound / 64):
+// the LDS a workgroup reserves is QW queues.
 
 // Per-wave queue of ambiguous samples awaiting FP64 re-evaluation.
 constexpr int kAmbCap = 512;
@@ -210,11 +210,11 @@ struct AmbQueue {
 // stalling 63 lanes behind each one.  Each lane adds those terms in sample
 // order into Ta, so the result has the same bits as the inline form.  Under
 // audit the re-evaluations are made inline.
-template <int B>
+template <int B, int QW>
 __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, int i, int j,
                                                   bool valid, RayCounts& rc) {
-  __shared__ AmbQueue queues[kQueueWaves];
-  AmbQueue& Q = queues[threadIdx.x >> 6];
+  __shared__ AmbQueue queues[QW];
+  AmbQueue& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
   const int lane = threadIdx.x & 63;
   // per-ray counts fit 32 bits (a ray has fewer than 2^31 samples)
   unsigned n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
@@ -468,12 +468,12 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
 // Optical depth T of pixel (i, j) of view v, hierarchical integrator.  Every
 // lane of the wave calls it together (lanes without a pixel pass valid =
 // false and still help drain the queue); T is meaningful for valid lanes.
-template <int PREC>
+template <int PREC, int QW>
 __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int i, int j,
                                                bool valid, RayCounts& rc) {
   constexpr int B = PREC == PREC_FP64 ? 1 : kHierBlock;
-  __shared__ HierQueue queues[kQueueWaves];
-  HierQueue& Q = queues[threadIdx.x >> 6];
+  __shared__ HierQueue queues[QW];
+  HierQueue& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
   const int lane = threadIdx.x & 63;
   const double DS = p.ds, dsf = p.ds_fine, dm = p.dm;
   unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0, n_skip = 0;
@@ -655,16 +655,16 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
 // T of pixel (i, j) of view v.  Every lane of the wave calls it together;
 // lanes with valid = false have no pixel and their return value is unused.
 // B: samples per node visit of the guarded simple march; DEFER: queue its
-// FP64 re-evaluations per wave (same result).
-template <int PREC, int INTEG, int B = kBlock, bool DEFER = false>
+// FP64 re-evaluations per wave (same result); QW: waves per workgroup.
+template <int PREC, int INTEG, int QW, int B = kBlock, bool DEFER = false>
 __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
   if constexpr (INTEG == INT_HIERARCHICAL) {
-    return march_T_hier<PREC>(p, v, i, j, valid, rc);
+    return march_T_hier<PREC, QW>(p, v, i, j, valid, rc);
   } else if constexpr (PREC == PREC_FP64) {
     return valid ? march_T_simple64(p, v, i, j, rc) : 0.0;
   } else if constexpr (DEFER) {
-    return march_T_guarded<B>(p, v, i, j, valid, rc);
+    return march_T_guarded<B, QW>(p, v, i, j, valid, rc);
   } else {
     return valid ? march_T_guarded_inline<B>(p, v, i, j, rc) : 0.0;
   }
@@ -672,10 +672,10 @@ __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, in
 
 // March pixel (i, j) of view v and store its transmittance (the image is
 // written at out + v * out_view_stride).  Called by every lane of the wave.
-template <int PREC, int INTEG, int B = kBlock, bool DEFER = false>
+template <int PREC, int INTEG, int QW, int B = kBlock, bool DEFER = false>
 __device__ __forceinline__ void march_ray(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
-  const double T = march_T<PREC, INTEG, B, DEFER>(p, v, i, j, valid, rc);
+  const double T = march_T<PREC, INTEG, QW, B, DEFER>(p, v, i, j, valid, rc);
   if (valid) store_pixel(p, v, i, j, T);
 }
 
