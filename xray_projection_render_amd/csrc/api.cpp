@@ -26,6 +26,27 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// Pinned host staging buffer (hipHostMalloc), grown on demand.
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -57,6 +78,10 @@ struct xrp_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::mutex mu;
   DevBuf views, table, stats, out, nodes, defs;
+  // per-render upload (nodes | deformations | views | zeroed stats), staged
+  // through pinned memory and sent with one copy
+  DevBuf upload;
+  HostBuf staging;
   // sample table cache
   int table_mode = -1;
   double table_smin = NAN, table_smax = NAN, table_ds = NAN;
@@ -147,8 +172,9 @@ void xrp_context_destroy(xrp_context* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->views, &ctx->table, &ctx->stats, &ctx->out,
-                    &ctx->nodes, &ctx->defs})
+                    &ctx->nodes, &ctx->defs, &ctx->upload})
     b->release();
+  ctx->staging.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -623,28 +649,32 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     ctx->table_ds = job.ds;
   }
 
-  HIPCHK(ctx->nodes.ensure(pl.nodes.size() * sizeof(DevNode)));
-  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, s.ddefs.size()) * sizeof(DevDeform)));
-  HIPCHK(ctx->views.ensure(nv * sizeof(DevView)));
-  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
-  HIPCHK(hipMemcpyAsync(ctx->nodes.p, pl.nodes.data(),
-                        pl.nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice,
-                        ctx->stream));
-  if (!s.ddefs.empty())
-    HIPCHK(hipMemcpyAsync(ctx->defs.p, s.ddefs.data(),
-                          s.ddefs.size() * sizeof(DevDeform),
-                          hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemcpyAsync(ctx->views.p, job.views.data(), nv * sizeof(DevView),
-                        hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long),
-                        ctx->stream));
+  // One pinned staging copy of nodes, deformations, views and zeroed
+  // stats (256-byte aligned sections), sent with one copy: the per-call
+  // host cost matters for small renders.
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t b_nodes = pl.nodes.size() * sizeof(DevNode);
+  const size_t b_defs = s.ddefs.size() * sizeof(DevDeform);
+  const size_t b_views = (size_t)nv * sizeof(DevView);
+  const size_t b_stats = ST_COUNT * sizeof(unsigned long long);
+  const size_t o_defs = align(b_nodes), o_views = o_defs + align(std::max<size_t>(b_defs, 1));
+  const size_t o_stats = o_views + align(b_views), total = o_stats + b_stats;
+  HIPCHK(ctx->staging.ensure(total));
+  HIPCHK(ctx->upload.ensure(total));
+  unsigned char* hb = (unsigned char*)ctx->staging.p;
+  std::memcpy(hb, pl.nodes.data(), b_nodes);
+  if (b_defs) std::memcpy(hb + o_defs, s.ddefs.data(), b_defs);
+  std::memcpy(hb + o_views, job.views.data(), b_views);
+  std::memset(hb + o_stats, 0, b_stats);
+  HIPCHK(hipMemcpyAsync(ctx->upload.p, hb, total, hipMemcpyHostToDevice, ctx->stream));
+  unsigned char* db = (unsigned char*)ctx->upload.p;
 
   MarchParams mp = pl.mp;
-  mp.nodes = (const DevNode*)ctx->nodes.p;
-  mp.defs = (const DevDeform*)ctx->defs.p;
-  mp.views = (const DevView*)ctx->views.p;
+  mp.nodes = (const DevNode*)db;
+  mp.defs = (const DevDeform*)(db + o_defs);
+  mp.views = (const DevView*)(db + o_views);
   mp.s_table = (const double*)ctx->table.p;
-  mp.stats = (unsigned long long*)ctx->stats.p;
+  mp.stats = (unsigned long long*)(db + o_stats);
   (void)res;
 
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
@@ -661,15 +691,21 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
                                                              : INT_SIMPLE;
     int e = vox_kernel ? launch_voxel(mp, layout, ctx->stream)
                        : launch_march(mp, prec, integ, ctx->stream);
-    if (e != 0)
+    if (e != 0) {
+      // the staging copy may still be reading the pinned buffer
+      (void)hipStreamSynchronize(ctx->stream);
       return fail(XRP_ERR_DEVICE, std::string("kernel launch failed: ") +
                                       hipGetErrorString((hipError_t)e));
+    }
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-  unsigned long long hst[ST_COUNT];
-  HIPCHK(hipMemcpyAsync(hst, ctx->stats.p, sizeof hst, hipMemcpyDeviceToHost,
-                        ctx->stream));
+  // the staging copy above has completed in stream order: reuse its stats
+  // section for the read-back
+  unsigned long long* hstp = (unsigned long long*)(hb + o_stats);
+  HIPCHK(hipMemcpyAsync(hstp, mp.stats, b_stats, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  unsigned long long hst[ST_COUNT];
+  std::memcpy(hst, hstp, sizeof hst);
   if (st) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
