@@ -270,9 +270,9 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
       root_blk<B, true>(p.nodes, P, live, vv, amb, dist);
       n_reeval += (unsigned)__builtin_popcount(amb);
       if (p.audit) {
-        // FP64 inline: the ambiguous samples, or under audit every sample
-        // (the FP32-clear ones must agree)
-        unsigned redo = p.audit ? live : amb;
+        // under audit every sample in FP64, inline (the FP32-clear ones
+        // must agree)
+        unsigned redo = live;
         while (redo) {
           const int q = __builtin_ctz(redo);
           redo &= redo - 1u;
