@@ -543,7 +543,10 @@ __device__ __forceinline__ bool sur_domain(const SurNet& net, float x, float y, 
   return ux >= 0.0f && ux <= 1.0f && uy >= 0.0f && uy <= 1.0f && uz >= 0.0f && uz <= 1.0f;
 }
 
-__global__ __launch_bounds__(256) void sur_render_kernel(SurNet net, MarchParams p,
+// One wave per workgroup (16 rays, a 4x4 pixel tile, each with its own LDS
+// copy of the weights): waves whose rays end early free their slot at once
+// (5 % faster than 8x8 tiles of four waves).
+__global__ __launch_bounds__(64) void sur_render_kernel(SurNet net, MarchParams p,
                                                          double ds_sur, long K) {
   __shared__ SurFwdLds F;
   const SurOffsets off = sur_offsets(net.log2_table);
@@ -555,12 +558,12 @@ __global__ __launch_bounds__(256) void sur_render_kernel(SurNet net, MarchParams
   const float* w3 = P + off.w3;
   const float b3 = P[off.b3];
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int s = lane & 15, g = lane >> 4;
-  const int tiles_j = (p.res + 7) >> 3;
+  const int tiles_j = (p.res + 3) >> 2;
   const int ti = blockIdx.x / tiles_j, tj = blockIdx.x - ti * tiles_j;
-  const int i = ti * 8 + (wave >> 1) * 4 + (s >> 2);
-  const int j = tj * 8 + (wave & 1) * 4 + (s & 3);
+  const int i = ti * 4 + (s >> 2);
+  const int j = tj * 4 + (s & 3);
   const int v = blockIdx.y;
   const bool valid = i < p.res && j < p.res;
 
@@ -708,8 +711,8 @@ int launch_sur_predict(const SurNet& net, const float* pts, int64_t n, float* ou
 
 int launch_sur_render(const SurNet& net, const MarchParams& p, double ds_sur, void* stream) {
   const long K = (long)std::ceil((p.smax - p.smin) / ds_sur);
-  const int tiles = (p.res + 7) / 8;
-  hipLaunchKernelGGL(sur_render_kernel, dim3(tiles * tiles, p.n_views), dim3(256), 0,
+  const int tiles = (p.res + 3) / 4;
+  hipLaunchKernelGGL(sur_render_kernel, dim3(tiles * tiles, p.n_views), dim3(64), 0,
                      (hipStream_t)stream, net, p, ds_sur, K);
   return (int)hipGetLastError();
 }
