@@ -32,6 +32,67 @@ struct PBlock {
   float x[S], y[S], z[S];
 };
 
+// Safe-radius bounds of single points, for the neighbouring cell that
+// tess_blk looks into: the same bounds and slacks as the blocked
+// evaluators apply to their block's last sample.
+__device__ __forceinline__ float sphere_dist(const DevNode& n, float x, float y, float z) {
+  const float dx = x - n.f[0], dy = y - n.f[1], dz = z - n.f[2];
+  const float l = __builtin_amdgcn_sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+  return fabsf(l - n.g[4]) - n.g[5];
+}
+__device__ __forceinline__ float box_dist(const DevNode& n, float x, float y, float z) {
+  const float ax = fabsf(x - n.f[0]) - n.f[3];
+  const float ay = fabsf(y - n.f[1]) - n.f[4];
+  const float az = fabsf(z - n.f[2]) - n.f[5];
+  return fabsf(fmaxf(ax, fmaxf(ay, az))) - n.g[5];
+}
+__device__ __forceinline__ float pped_dist(const DevNode& n, float x, float y, float z) {
+  const float w0 = x - n.f[0], w1 = y - n.f[1], w2 = z - n.f[2];
+  const float u = fmaf(n.f[3], w0, fmaf(n.f[6], w1, n.f[9] * w2));
+  const float v = fmaf(n.f[4], w0, fmaf(n.f[7], w1, n.f[10] * w2));
+  const float t = fmaf(n.f[5], w0, fmaf(n.f[8], w1, n.f[11] * w2));
+  const float e = fmaxf(fmaxf(fmaxf(-u, u - 1.0f), fmaxf(-v, v - 1.0f)), fmaxf(-t, t - 1.0f));
+  return (fabsf(e) - n.g[5]) * n.g[4];
+}
+__device__ __forceinline__ float cyl_dist(const DevNode& n, float x, float y, float z) {
+  const float w0 = x - n.f[0], w1 = y - n.f[1], w2 = z - n.f[2];
+  const float t = fmaf(w0, n.f[3], fmaf(w1, n.f[4], w2 * n.f[5]));
+  const float c = t * n.f[7];
+  const float e0 = fmaf(-c, n.f[3], w0), e1 = fmaf(-c, n.f[4], w1), e2 = fmaf(-c, n.f[5], w2);
+  const float ra = __builtin_amdgcn_sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2))) - n.g[4];
+  const float aa = fmaxf(-t, t - n.f[6]) * n.g[6];
+  return fabsf(fmaxf(ra, aa)) - n.g[5];
+}
+__device__ __forceinline__ float gyroid_dist(const DevNode& n, float x, float y, float z) {
+  const float qx = (x - n.f[0]) * n.f[3], qy = (y - n.f[1]) * n.f[3],
+              qz = (z - n.f[2]) * n.f[3];
+  float sx, cxs, sy, cys, sz, czs;
+  sincosf(qx, &sx, &cxs);
+  sincosf(qy, &sy, &cys);
+  sincosf(qz, &sz, &czs);
+  const float a = fabsf(fmaf(sx, cys, fmaf(sy, czs, sz * cxs))) - n.f[4];
+  return (fabsf(a) - n.g[0]) * n.g[4];
+}
+// Smallest bound over a collection's children (no skipping past a child
+// without one: nested tessellations and voxel grids give 0).
+__device__ __forceinline__ float coll_dist(const DevNode* __restrict__ N, int i, float x,
+                                           float y, float z) {
+  const DevNode& n = N[i];
+  float d = INFINITY;
+  for (int c = n.first; c < n.next; c = N[c].next) {
+    const DevNode& k = N[c];
+    switch (k.kind) {
+      case K_SPHERE: d = fminf(d, sphere_dist(k, x, y, z)); break;
+      case K_BOX: d = fminf(d, box_dist(k, x, y, z)); break;
+      case K_PPED: d = fminf(d, pped_dist(k, x, y, z)); break;
+      case K_CYL: d = fminf(d, cyl_dist(k, x, y, z)); break;
+      case K_GYROID: d = fminf(d, gyroid_dist(k, x, y, z)); break;
+      default: return 0.0f;
+    }
+  }
+  return d;
+}
+
 template <int S, bool DIST>
 __device__ __forceinline__ void sphere_blk(const DevNode& n, const PBlock<S>& p,
                                            double (&v)[S], unsigned& amb, float& dist) {
@@ -209,15 +270,45 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
     }
     if constexpr (DIST) {
       if (s == S - 1) {
-        // the bounds box (either side) and, inside, the cell's seams:
-        // a neighbouring cell holds the same content translated, which
-        // the unit cell's own distances do not see
+        // The bounds box (either side); inside it, the content of the
+        // nearest neighbouring cell, which the unit cell's own distances
+        // (from the walk below) do not see, and the second-nearest face:
+        // a ball smaller than that meets only this cell and the neighbour
+        // across the nearest face.  Face distances are lower bounds (the
+        // fractions are within the band gw).
         const float eb = fmaxf(fmaxf(fmaxf(b0 - x, x - b1), fmaxf(b2 - y, y - b3)),
                                fmaxf(b4 - z, z - b5));
-        const float sf = fminf(fminf(fminf(rx, 1.0f - rx), fminf(ry, 1.0f - ry)),
-                               fminf(rz, 1.0f - rz));
         dist = fminf(dist, fabsf(eb) - gb);
-        if (!o) dist = fminf(dist, (sf - gw) * n.g[4]);
+        if (!o) {
+          const float sx = n.f[9], sy = n.f[10], sz = n.f[11];
+          const float fc[6] = {(rx - gw) * sx, (1.0f - rx - gw) * sx, (ry - gw) * sy,
+                               (1.0f - ry - gw) * sy, (rz - gw) * sz, (1.0f - rz - gw) * sz};
+          int a = 0;
+          float f1 = fc[0], f2 = INFINITY;
+#pragma unroll
+          for (int t = 1; t < 6; ++t) {
+            if (fc[t] < f1) {
+              f2 = f1;
+              f1 = fc[t];
+              a = t;
+            } else {
+              f2 = fminf(f2, fc[t]);
+            }
+          }
+          // neighbour across face a: its content sits one cell side lower
+          // (a even) or higher (a odd), i.e. this point one side up or down
+          const float sh = (a & 1) ? -1.0f : 1.0f;
+          float nx2 = q.x[s], ny2 = q.y[s], nz2 = q.z[s];
+          if (a < 2)
+            nx2 = fmaf(sh, sx, nx2);
+          else if (a < 4)
+            ny2 = fmaf(sh, sy, ny2);
+          else
+            nz2 = fmaf(sh, sz, nz2);
+          float dn = 0.0f;
+          if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, nx2, ny2, nz2);
+          dist = fminf(dist, fminf(f2, dn));
+        }
       }
     }
   }

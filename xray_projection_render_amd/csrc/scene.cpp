@@ -1182,7 +1182,6 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         n.g[1] = (float)(kappa * (d + u * (bb + bm)));
         double F = (bb + um) / dxmin;
         n.g[0] = (float)(kappa * ((d + u * um + 2 * u * (bb + um)) / dxmin + 2 * u * F));
-        n.g[4] = (float)(dxmin * (1 - 1e-6));  // seam distance per unit fraction
         dl[n.depth + 1] = d + 4 * u * (bb + um + dxmax);
         Bl[n.depth + 1] = um;
         break;
