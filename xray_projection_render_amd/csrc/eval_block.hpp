@@ -241,6 +241,10 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
   const float gb = n.g[1], gw = n.g[0];
   PBlock<S> q;
   unsigned go = 0, amb = 0;
+  // DIST, last sample inside the bounds: nearest and second-nearest cell
+  // face and the point as seen from the neighbour across the nearest one
+  bool nb = false;
+  float nb_f1 = 0.0f, nb_f2 = 0.0f, nb_x = 0.0f, nb_y = 0.0f, nb_z = 0.0f;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const float x = p.x[s], y = p.y[s], z = p.z[s];
@@ -270,12 +274,9 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
     }
     if constexpr (DIST) {
       if (s == S - 1) {
-        // The bounds box (either side); inside it, the content of the
-        // nearest neighbouring cell, which the unit cell's own distances
-        // (from the walk below) do not see, and the second-nearest face:
-        // a ball smaller than that meets only this cell and the neighbour
-        // across the nearest face.  Face distances are lower bounds (the
-        // fractions are within the band gw).
+        // The bounds box (either side); inside it, see after the walk.
+        // Face distances are lower bounds (the fractions are within the
+        // band gw).
         const float eb = fmaxf(fmaxf(fmaxf(b0 - x, x - b1), fmaxf(b2 - y, y - b3)),
                                fmaxf(b4 - z, z - b5));
         dist = fminf(dist, fabsf(eb) - gb);
@@ -298,16 +299,18 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
           // neighbour across face a: its content sits one cell side lower
           // (a even) or higher (a odd), i.e. this point one side up or down
           const float sh = (a & 1) ? -1.0f : 1.0f;
-          float nx2 = q.x[s], ny2 = q.y[s], nz2 = q.z[s];
+          nb_x = q.x[s];
+          nb_y = q.y[s];
+          nb_z = q.z[s];
           if (a < 2)
-            nx2 = fmaf(sh, sx, nx2);
+            nb_x = fmaf(sh, sx, nb_x);
           else if (a < 4)
-            ny2 = fmaf(sh, sy, ny2);
+            nb_y = fmaf(sh, sy, nb_y);
           else
-            nz2 = fmaf(sh, sz, nz2);
-          float dn = 0.0f;
-          if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, nx2, ny2, nz2);
-          dist = fminf(dist, fminf(f2, dn));
+            nb_z = fmaf(sh, sz, nb_z);
+          nb = true;
+          nb_f1 = f1;
+          nb_f2 = f2;
         }
       }
     }
@@ -319,6 +322,17 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
     if (go) coll_blk<D + 1, S, DIST>(N, n.first, q, go, out, amb_out, dist);
   } else {
     amb_out |= go;
+  }
+  if constexpr (DIST) {
+    // A radius up to the nearest face stays in this cell, whose content the
+    // walk above bounded.  Beyond it, the neighbour across that face holds
+    // the same content translated, and up to the second-nearest face a ball
+    // meets only these two cells.
+    if (nb && dist > nb_f1) {
+      float dn = 0.0f;
+      if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, nb_x, nb_y, nb_z);
+      dist = fminf(dist, fminf(nb_f2, dn));
+    }
   }
 }
 
