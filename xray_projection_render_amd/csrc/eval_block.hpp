@@ -241,10 +241,6 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
   const float gb = n.g[1], gw = n.g[0];
   PBlock<S> q;
   unsigned go = 0, amb = 0;
-  // DIST, last sample inside the bounds: nearest and second-nearest cell
-  // face and the point as seen from the neighbour across the nearest one
-  bool nb = false;
-  float nb_f1 = 0.0f, nb_f2 = 0.0f, nb_x = 0.0f, nb_y = 0.0f, nb_z = 0.0f;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const float x = p.x[s], y = p.y[s], z = p.z[s];
@@ -274,44 +270,10 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
     }
     if constexpr (DIST) {
       if (s == S - 1) {
-        // The bounds box (either side); inside it, see after the walk.
-        // Face distances are lower bounds (the fractions are within the
-        // band gw).
+        // the bounds box, either side (the cell faces: after the walk)
         const float eb = fmaxf(fmaxf(fmaxf(b0 - x, x - b1), fmaxf(b2 - y, y - b3)),
                                fmaxf(b4 - z, z - b5));
         dist = fminf(dist, fabsf(eb) - gb);
-        if (!o) {
-          const float sx = n.f[9], sy = n.f[10], sz = n.f[11];
-          const float fc[6] = {(rx - gw) * sx, (1.0f - rx - gw) * sx, (ry - gw) * sy,
-                               (1.0f - ry - gw) * sy, (rz - gw) * sz, (1.0f - rz - gw) * sz};
-          int a = 0;
-          float f1 = fc[0], f2 = INFINITY;
-#pragma unroll
-          for (int t = 1; t < 6; ++t) {
-            if (fc[t] < f1) {
-              f2 = f1;
-              f1 = fc[t];
-              a = t;
-            } else {
-              f2 = fminf(f2, fc[t]);
-            }
-          }
-          // neighbour across face a: its content sits one cell side lower
-          // (a even) or higher (a odd), i.e. this point one side up or down
-          const float sh = (a & 1) ? -1.0f : 1.0f;
-          nb_x = q.x[s];
-          nb_y = q.y[s];
-          nb_z = q.z[s];
-          if (a < 2)
-            nb_x = fmaf(sh, sx, nb_x);
-          else if (a < 4)
-            nb_y = fmaf(sh, sy, nb_y);
-          else
-            nb_z = fmaf(sh, sz, nb_z);
-          nb = true;
-          nb_f1 = f1;
-          nb_f2 = f2;
-        }
       }
     }
   }
@@ -324,14 +286,50 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
     amb_out |= go;
   }
   if constexpr (DIST) {
-    // A radius up to the nearest face stays in this cell, whose content the
-    // walk above bounded.  Beyond it, the neighbour across that face holds
-    // the same content translated, and up to the second-nearest face a ball
-    // meets only these two cells.
-    if (nb && dist > nb_f1) {
+    // Last sample inside the bounds: a radius up to the nearest cell face
+    // stays in this cell, whose content the walk above bounded.  Beyond it,
+    // the neighbour across that face holds the same content translated, and
+    // up to the second-nearest face a ball meets only these two cells.  The
+    // wrap is recomputed here (the same arithmetic as above) rather than
+    // kept live through the walk.  Face distances are lower bounds (the
+    // fractions are within the band gw).
+    const float x = p.x[S - 1], y = p.y[S - 1], z = p.z[S - 1];
+    const bool o = x < b0 - gb || x > b1 + gb || y < b2 - gb || y > b3 + gb ||
+                   z < b4 - gb || z > b5 + gb;
+    const float sx = n.f[9], sy = n.f[10], sz = n.f[11];
+    const float fx = (x - n.f[6]) * n.f[12];
+    const float fy = (y - n.f[7]) * n.f[13];
+    const float fz = (z - n.f[8]) * n.f[14];
+    const float nx = floorf(fx), ny = floorf(fy), nz = floorf(fz);
+    const float rx = fx - nx, ry = fy - ny, rz = fz - nz;
+    const float fc[6] = {(rx - gw) * sx, (1.0f - rx - gw) * sx, (ry - gw) * sy,
+                         (1.0f - ry - gw) * sy, (rz - gw) * sz, (1.0f - rz - gw) * sz};
+    int a = 0;
+    float f1 = fc[0], f2 = INFINITY;
+#pragma unroll
+    for (int t = 1; t < 6; ++t) {
+      if (fc[t] < f1) {
+        f2 = f1;
+        f1 = fc[t];
+        a = t;
+      } else {
+        f2 = fminf(f2, fc[t]);
+      }
+    }
+    if (!o && dist > f1) {
+      // this cell's point, moved one cell side up (a even: the neighbour
+      // below) or down (a odd: the neighbour above)
+      const float sh = (a & 1) ? -1.0f : 1.0f;
+      float qx = fmaf(-sx, nx, x), qy = fmaf(-sy, ny, y), qz = fmaf(-sz, nz, z);
+      if (a < 2)
+        qx = fmaf(sh, sx, qx);
+      else if (a < 4)
+        qy = fmaf(sh, sy, qy);
+      else
+        qz = fmaf(sh, sz, qz);
       float dn = 0.0f;
-      if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, nb_x, nb_y, nb_z);
-      dist = fminf(dist, fminf(nb_f2, dn));
+      if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, qx, qy, qz);
+      dist = fminf(dist, fminf(f2, dn));
     }
   }
 }
