@@ -7,6 +7,7 @@ float32 rounding of the stored transmittance (<= 6e-8) and the audit
 counter (FP32-clear samples whose FP64 value differs) must be zero.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -136,6 +137,56 @@ def test_safe_radius_skipping(ctx, name):
                          audit=False)
     assert np.array_equal(img2, img)
     assert st2["skipped_samples"] == st["skipped_samples"]
+
+
+_NO_SKIP_SCRIPT = r"""
+import sys, numpy as np
+import xray_projection_render_amd as X
+from xray_projection_render_amd import workloads as W
+ctx = X.Context(0)
+out = {}
+for name in ("csg", "lattice", "gyroid"):
+    if name == "csg":
+        w, dfm = W.csg(resolution=64, n_views=2), None
+    elif name == "lattice":
+        w = W.lattice(resolution=64, n_views=2); dfm = w.deformation
+    else:
+        w = W.gyroid_lattice(resolution=48, n_views=2, ds=0.004, shear=True); dfm = w.deformation
+    sc = X.Scene.from_dict(w.obj, dfm)
+    for integ in ("simple", "hierarchical"):
+        img, st = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator=integ)
+        assert st["skipped_samples"] == 0
+        out[name + "_" + integ] = img
+    sc.close()
+ctx.close()
+np.savez(sys.argv[1], **out)
+"""
+
+
+def test_safe_radius_settling_changes_no_bits(ctx, tmp_path):
+    """The same renders with settling switched off (XRP_SKIP=0, read once per
+    process, hence a child process) give the same bits."""
+    import subprocess
+    import sys as _sys
+    path = tmp_path / "noskip.npz"
+    env = dict(os.environ, XRP_SKIP="0")
+    subprocess.run([_sys.executable, "-c", _NO_SKIP_SCRIPT, str(path)], env=env, check=True,
+                   timeout=300)
+    ref = np.load(path)
+    for name in ("csg", "lattice", "gyroid"):
+        if name == "csg":
+            w, dfm = W.csg(resolution=64, n_views=2), None
+        elif name == "lattice":
+            w = W.lattice(resolution=64, n_views=2)
+            dfm = w.deformation
+        else:
+            w = W.gyroid_lattice(resolution=48, n_views=2, ds=0.004, shear=True)
+            dfm = w.deformation
+        sc = xrp_scene(w.obj, dfm)
+        for integ in ("simple", "hierarchical"):
+            img, st = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator=integ)
+            assert st["skipped_samples"] > 0, (name, integ)
+            assert np.array_equal(img, ref[name + "_" + integ]), (name, integ)
 
 
 @pytest.mark.parametrize("name", ["csg", "lattice"])

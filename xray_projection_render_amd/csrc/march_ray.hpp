@@ -393,17 +393,18 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
 // evaluations, and on lattices some lane toggles at most coarse samples.
 // Here a toggle is appended to a per-wave queue in LDS; once the queue is
 // half full, or every lane has finished its coarse samples, the whole wave
-// drains it, one queued toggle per lane and kBlock fine samples per node
-// visit.  Each lane links its own toggles and adds their fine sums to its
-// T in a fixed order, so the result does not depend on timing; it differs
-// from the reference's only in the FP64 summation order.
+// drains it, one queued toggle per lane and kHierBlock fine samples per node
+// visit.  Each lane links its own toggles and adds their fine sums, in
+// coarse order, to a separate sum, so the result does not depend on when
+// the queue is drained; it differs from the reference's only in the FP64
+// summation order.
 
 constexpr int kHierCap = 512;   // queued toggles per wave
 
 struct HierQueue {
   double sum[kHierCap];     // fine sum of the step: sum of (rho * dm) * DS/10
   int32_t n[kHierCap];      // coarse index of the step's right endpoint
-  int16_t link[kHierCap];   // the owning lane's previous entry, -1 for none
+  int16_t link[kHierCap];   // the owning lane's next entry, -1 for none
   uint8_t owner[kHierCap];  // lane whose ray the step belongs to
 };
 
@@ -497,8 +498,13 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
     }
   }
   bool prev_zero = true;  // main.go: prev_rho starts at 0
-  int head = -1;          // this lane's newest queue entry
-  int queued = 0;         // entries in the wave's queue (same on every lane)
+  int head = -1, tail = -1;  // this lane's queue entries, in coarse order
+  int queued = 0;            // entries in the wave's queue (same on every lane)
+  // The coarse terms go into T and the fine sums, in coarse order, into
+  // Tf: the result, T + Tf, does not depend on when the queue is drained
+  // (nor, therefore, on which other rays share the wave or on how many
+  // samples the safe radius settles).
+  double Tf = 0.0;
   for (;;) {
     const int nb = n;
     unsigned tog = 0;
@@ -563,8 +569,12 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
         const int pos = queued + (int)__popcll(m & __lanemask_lt());
         Q.n[pos] = nb + q;
         Q.owner[pos] = (uint8_t)lane;
-        Q.link[pos] = (int16_t)head;
-        head = pos;
+        Q.link[pos] = -1;
+        if (tail >= 0)
+          Q.link[tail] = (int16_t)pos;
+        else
+          head = pos;
+        tail = pos;
       }
       queued += (int)__popcll(m);
     }
@@ -636,9 +646,10 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       while (head >= 0) {
-        T += Q.sum[head];
+        Tf += Q.sum[head];
         head = Q.link[head];
       }
+      tail = -1;
       queued = 0;
       __builtin_amdgcn_wave_barrier();
     }
@@ -649,7 +660,7 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
   rc.miss += n_miss;
   rc.fine += n_fine;
   rc.skipped += n_skip;
-  return T;
+  return T + Tf;
 }
 
 // T of pixel (i, j) of view v.  Every lane of the wave calls it together;
