@@ -436,11 +436,13 @@ def main() -> int:
         roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None}
     if w.voxels is not None and prec != "fp64" and w.integrator == "simple":
-        layout = os.environ.get("XRP_VOXEL_LAYOUT", "cached")
+        layout = os.environ.get("XRP_VOXEL_LAYOUT", "quad")
         roof["kernel"] = {"linear": "voxel_kernel<VOX_LINEAR,CACHE_NONE>",
                           "brick4": "voxel_kernel<VOX_BRICK4,CACHE_NONE>",
                           "cached": "voxel_kernel<VOX_BRICK4,CACHE_CELL>",
                           "linear_cached": "voxel_kernel<VOX_LINEAR,CACHE_CELL>",
+                          "pair": "voxel_kernel<VOX_BRICK4_PAIR,CACHE_CELL>",
+                          "quad": "voxel_kernel<VOX_BRICK4_QUAD,CACHE_CELL>",
                           }.get(layout, "voxel_lds_kernel")
     else:
         roof["kernel"] = "march_kernel<%s,%s>" % (

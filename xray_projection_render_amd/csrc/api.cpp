@@ -364,6 +364,35 @@ std::vector<float> brick4(const std::vector<float>& v, int nx, int ny, int nz) {
   return out;
 }
 
+// Brick-order copies with each voxel's upper-y neighbour (PAIR: v[y], v[y1])
+// or upper-y and upper-z neighbours (QUAD: v[z][y], v[z1][y], v[z][y1],
+// v[z1][y1]) stored beside it, upper indices clamped to N-1 as
+// objects.go:811-830 clamps them, so a cell's 8 corners come in 4 or 2 wide
+// loads.
+std::vector<float> brick4_wide(const std::vector<float>& v, int nx, int ny, int nz, int w) {
+  int bx = (nx + 3) / 4, by = (ny + 3) / 4, bz = (nz + 3) / 4;
+  std::vector<float> out((size_t)bx * by * bz * 64 * w, 0.0f);
+  auto at = [&](int z, int x, int y) { return v[((size_t)z * nx + x) * ny + y]; };
+  for (int z = 0; z < nz; ++z)
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y) {
+        size_t b = ((size_t)(z >> 2) * bx + (x >> 2)) * by + (y >> 2);
+        size_t l = (size_t)(z & 3) * 16 + (x & 3) * 4 + (y & 3);
+        float* o = &out[(b * 64 + l) * w];
+        const int y1 = std::min(y + 1, ny - 1), z1 = std::min(z + 1, nz - 1);
+        if (w == 2) {
+          o[0] = at(z, x, y);
+          o[1] = at(z, x, y1);
+        } else {
+          o[0] = at(z, x, y);
+          o[1] = at(z1, x, y);
+          o[2] = at(z, x, y1);
+          o[3] = at(z1, x, y1);
+        }
+      }
+  return out;
+}
+
 // World -> object map of a chain of affine/linear/rigid deformations.
 bool compose_linear(const std::vector<xrp::HDeform>& defs, double M[9], double t[3]) {
   double m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, v[3] = {0, 0, 0};
@@ -394,53 +423,64 @@ bool compose_linear(const std::vector<xrp::HDeform>& defs, double M[9], double t
   return true;
 }
 
-int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs) {
+// Device copies of a scene's voxel grids on one context: the linear copy of
+// every voxel node (preorder, matching the flattened node order), then, for
+// a voxel root, three slots for the brick-order copies of width 1, 2 and 4
+// that the voxel-root kernel reads.  A brick slot is filled on first use
+// (brick_width 1, 2 or 4; 0 asks for none), so a scene holds only the
+// copies its layouts need.
+int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs,
+                  int brick_width = 0) {
   std::lock_guard<std::mutex> g(g_registry_mu);
-  auto it = sc->uploads.find(ctx);
-  if (it != sc->uploads.end()) {
-    ptrs = it->second;
-    return XRP_OK;
-  }
-  std::vector<const xrp::HNode*> vox;
-  // preorder, matching the flattened node order
-  std::function<void(const xrp::HNode&)> walk = [&](const xrp::HNode& n) {
-    if (n.kind == xrp::K_VOXEL) vox.push_back(&n);
-    for (auto& k : n.kids) walk(k);
-  };
-  walk(sc->s->root);
-  std::vector<void*> out;
-  for (auto* n : vox) {
-    void* p = nullptr;
-    size_t bytes = n->vox->size() * sizeof(float);
+  const bool root_vox = sc->s->root.kind == xrp::K_VOXEL;
+  auto put = [](const std::vector<float>& host, size_t bytes, void*& p) -> hipError_t {
+    p = nullptr;
     hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 4));
-    if (e == hipSuccess)
-      e = hipMemcpy(p, n->vox->data(), bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      if (p) (void)hipFree(p);
-      for (void* q : out) (void)hipFree(q);
-      return fail(XRP_ERR_ALLOC, std::string("voxel upload failed: ") +
-                                     hipGetErrorString(e));
+    if (e == hipSuccess) e = hipMemcpy(p, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess && p) {
+      (void)hipFree(p);
+      p = nullptr;
     }
-    out.push_back(p);
-  }
-  if (sc->s->root.kind == xrp::K_VOXEL) {
-    // 4x4x4-brick copy for the voxel-root kernel (appended last).
-    const xrp::HNode& n = sc->s->root;
-    std::vector<float> br = brick4(*n.vox, n.nx, n.ny, n.nz);
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, br.size() * sizeof(float));
-    if (e == hipSuccess)
-      e = hipMemcpy(p, br.data(), br.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      if (p) (void)hipFree(p);
-      for (void* q : out) (void)hipFree(q);
-      return fail(XRP_ERR_ALLOC, std::string("voxel upload failed: ") +
-                                     hipGetErrorString(e));
+    return e;
+  };
+  auto it = sc->uploads.find(ctx);
+  if (it == sc->uploads.end()) {
+    std::vector<const xrp::HNode*> vox;
+    std::function<void(const xrp::HNode&)> walk = [&](const xrp::HNode& n) {
+      if (n.kind == xrp::K_VOXEL) vox.push_back(&n);
+      for (auto& k : n.kids) walk(k);
+    };
+    walk(sc->s->root);
+    std::vector<void*> out;
+    for (auto* n : vox) {
+      void* p = nullptr;
+      hipError_t e = put(*n->vox, n->vox->size() * sizeof(float), p);
+      if (e != hipSuccess) {
+        for (void* q : out) (void)hipFree(q);
+        return fail(XRP_ERR_ALLOC, std::string("voxel upload failed: ") +
+                                       hipGetErrorString(e));
+      }
+      out.push_back(p);
     }
-    out.push_back(p);
+    if (root_vox) out.insert(out.end(), 3, nullptr);
+    it = sc->uploads.emplace(ctx, std::move(out)).first;
+    ctx->scenes.insert(sc);
   }
-  sc->uploads[ctx] = out;
-  ctx->scenes.insert(sc);
+  std::vector<void*>& out = it->second;
+  if (root_vox && brick_width) {
+    const int slot = brick_width == 1 ? 0 : brick_width == 2 ? 1 : 2;
+    void*& p = out[out.size() - 3 + slot];
+    if (!p) {
+      const xrp::HNode& n = sc->s->root;
+      std::vector<float> br = brick_width == 1
+                                  ? brick4(*n.vox, n.nx, n.ny, n.nz)
+                                  : brick4_wide(*n.vox, n.nx, n.ny, n.nz, brick_width);
+      hipError_t e = put(br, br.size() * sizeof(float), p);
+      if (e != hipSuccess)
+        return fail(XRP_ERR_ALLOC, std::string("voxel upload failed: ") +
+                                       hipGetErrorString(e));
+    }
+  }
   ptrs = out;
   return XRP_OK;
 }
@@ -460,7 +500,7 @@ struct LaunchPlan {
   std::vector<double> table;
   int precision = XRP_PRECISION_GUARDED;
   bool vox_kernel = false;
-  int layout = xrp::VOX_BRICK4_CACHED;
+  int layout = xrp::VOX_BRICK4_QUAD;
   double delta = 0.0;
 };
 
@@ -477,7 +517,7 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       allow_voxel_kernel && s.root.kind == K_VOXEL && compose_linear(s.defs, vM, vt) &&
       job.integrator == XRP_INTEGRATE_SIMPLE && !job.audit &&
       job.precision != XRP_PRECISION_FP64 &&
-      (size_t)mp_bricks(s.root) * 64 < ((size_t)1 << 30);
+      (size_t)mp_bricks(s.root) * 64 < ((size_t)1 << 28);
   int precision = job.precision;
   if (precision == XRP_PRECISION_AUTO)
     precision = (s.has_voxels && !vox_kernel) ? XRP_PRECISION_FP64 : XRP_PRECISION_GUARDED;
@@ -572,11 +612,12 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   // amortises (1-200 primitives: 4 is 14 % faster than 2).
   mp.block = s.nodes.size() >= 24 ? 4 : 2;
   mp.n_views = (int)job.views.size();
-  // 4^3-brick gathers with the per-lane cell cache measured fastest on the
-  // 256^3 voxel workload (4.1k projections/s; 3.0k uncached bricks, 2.8k
-  // cached linear, 2.2k LDS-staged, 1.6k linear); the others stay
-  // selectable through XRP_VOXEL_LAYOUT for comparison.
-  int layout = VOX_BRICK4_CACHED;
+  // 4^3 bricks of corner quads (two 16-byte loads per cell) with the
+  // per-lane cell cache measured fastest on the 256^3 voxel workload (9.85k
+  // 512^2 projections/s; plain bricks 7.94k, corner pairs 7.92k, uncached
+  // bricks 5.4k, cached linear 4.8k, LDS-staged 2.2k, linear 1.8k); the
+  // others stay selectable through XRP_VOXEL_LAYOUT for comparison.
+  int layout = VOX_BRICK4_QUAD;
   if (vox_kernel) {
     const char* env = std::getenv("XRP_VOXEL_LAYOUT");
     if (env) {
@@ -586,11 +627,20 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       if (e == "lds") layout = VOX_LDS;
       if (e == "cached") layout = VOX_BRICK4_CACHED;
       if (e == "linear_cached") layout = VOX_LINEAR_CACHED;
+      if (e == "pair") layout = VOX_BRICK4_PAIR;
+      if (e == "quad") layout = VOX_BRICK4_QUAD;
     }
     const bool lin = voxel_layout_linear(layout);
     const size_t nlin = (size_t)s.root.nx * s.root.ny * s.root.nz;
-    mp.vgrid = (const float*)(lin ? vptr.front() : vptr.back());
-    mp.vbytes = (uint32_t)(4ull * (lin ? nlin : (size_t)mp_bricks(s.root) * 64));
+    const int width = layout == VOX_BRICK4_PAIR ? 2 : layout == VOX_BRICK4_QUAD ? 4 : 1;
+    if (!lin) {
+      int rc = upload_voxels(ctx, sc, vptr, width);
+      if (rc) return rc;
+    }
+    const size_t nb = vptr.size();
+    mp.vgrid = (const float*)(lin ? vptr.front()
+                                  : vptr[nb - 3 + (width == 1 ? 0 : width == 2 ? 1 : 2)]);
+    mp.vbytes = (uint32_t)(4ull * (lin ? nlin : (size_t)mp_bricks(s.root) * 64 * width));
     mp.vlin = (const float*)vptr.front();
     mp.vlin_bytes = (uint32_t)(4ull * nlin);
     const HNode& vn = s.root;

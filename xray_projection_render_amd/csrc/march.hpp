@@ -53,7 +53,7 @@ struct MarchParams {
   int32_t out_view_stride; // floats between consecutive views in out
   // Voxel-root kernel: world -> object affine map (the composed linear
   // deformation chain), grid dims, brick layout and the grid-space guard.
-  const float* vgrid;      // linear [z][x][y] or 4^3-brick layout
+  const float* vgrid;      // linear [z][x][y], or 4^3 bricks of 1, 2 or 4 floats
   int32_t vn[3];           // nx ny nz
   int32_t vbricks[3];      // bricks per axis (x, y, z)
   double M[9];             // row-major
@@ -80,12 +80,17 @@ struct MarchParams {
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with
 // the 4^3-brick copy as the gather fallback for oversized sub-volumes.
 // *_CACHED: per-lane register cache of the last cell's 8 corners.
+// VOX_BRICK4_PAIR / VOX_BRICK4_QUAD (cached): brick elements of 2 floats
+// (v[y], v[y+1]) or 4 floats (v[z][y], v[z+1][y], v[z][y+1], v[z+1][y+1]),
+// upper indices clamped, so a cell's corners come in 4 or 2 loads.
 enum VoxelLayout : int32_t {
   VOX_LINEAR = 0,
   VOX_BRICK4 = 1,
   VOX_LDS = 2,
   VOX_BRICK4_CACHED = 3,
-  VOX_LINEAR_CACHED = 4
+  VOX_LINEAR_CACHED = 4,
+  VOX_BRICK4_PAIR = 5,
+  VOX_BRICK4_QUAD = 6
 };
 inline bool voxel_layout_linear(int layout) {
   return layout == VOX_LINEAR || layout == VOX_LINEAR_CACHED;

@@ -67,9 +67,33 @@ __device__ __forceinline__ float grid_load(__amdgpu_buffer_rsrc_t rs, unsigned e
 template <int LAYOUT>
 __device__ __forceinline__ void load_cell(const MarchParams& p, __amdgpu_buffer_rsrc_t rs,
                                           int x0, int y0, int z0, float (&c)[8]) {
-  const CellOffsets o = cell_offsets<LAYOUT>(p, x0, y0, z0);
+  if constexpr (LAYOUT == VOX_BRICK4_PAIR) {
+    const CellOffsets o = cell_offsets<VOX_BRICK4>(p, x0, y0, z0);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) c[q] = grid_load(rs, o.z[q & 1] + o.x[q >> 2] + o.y[(q >> 1) & 1]);
+    for (int q = 0; q < 4; ++q) {
+      const int dz = q & 1, dx = q >> 1;
+      const auto v = __builtin_amdgcn_raw_buffer_load_b64(
+          rs, (int)((o.z[dz] + o.x[dx] + o.y[0]) * 8u), 0, 0);
+      c[4 * dx + dz] = __uint_as_float(v[0]);
+      c[4 * dx + 2 + dz] = __uint_as_float(v[1]);
+    }
+  } else if constexpr (LAYOUT == VOX_BRICK4_QUAD) {
+    const CellOffsets o = cell_offsets<VOX_BRICK4>(p, x0, y0, z0);
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+          rs, (int)((o.z[0] + o.x[dx] + o.y[0]) * 16u), 0, 0);
+      c[4 * dx + 0] = __uint_as_float(v[0]);
+      c[4 * dx + 1] = __uint_as_float(v[1]);
+      c[4 * dx + 2] = __uint_as_float(v[2]);
+      c[4 * dx + 3] = __uint_as_float(v[3]);
+    }
+  } else {
+    const CellOffsets o = cell_offsets<LAYOUT>(p, x0, y0, z0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      c[q] = grid_load(rs, o.z[q & 1] + o.x[q >> 2] + o.y[(q >> 1) & 1]);
+  }
 }
 
 // Lerp order z, then y, then x (objects.go:845-851).
@@ -541,6 +565,12 @@ int launch_voxel(const MarchParams& p, int layout, void* stream) {
       break;
     case VOX_BRICK4_CACHED:
       hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4, CACHE_CELL>), grid, block, 0, s, p);
+      break;
+    case VOX_BRICK4_PAIR:
+      hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4_PAIR, CACHE_CELL>), grid, block, 0, s, p);
+      break;
+    case VOX_BRICK4_QUAD:
+      hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4_QUAD, CACHE_CELL>), grid, block, 0, s, p);
       break;
     case VOX_LINEAR_CACHED:
       hipLaunchKernelGGL((voxel_kernel<VOX_LINEAR, CACHE_CELL>), grid, block, 0, s, p);
