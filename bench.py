@@ -436,13 +436,14 @@ def main() -> int:
         roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None}
     if w.voxels is not None and prec != "fp64" and w.integrator == "simple":
-        layout = os.environ.get("XRP_VOXEL_LAYOUT", "quad")
+        layout = os.environ.get("XRP_VOXEL_LAYOUT", "quad_slide")
         roof["kernel"] = {"linear": "voxel_kernel<VOX_LINEAR,CACHE_NONE>",
                           "brick4": "voxel_kernel<VOX_BRICK4,CACHE_NONE>",
                           "cached": "voxel_kernel<VOX_BRICK4,CACHE_CELL>",
                           "linear_cached": "voxel_kernel<VOX_LINEAR,CACHE_CELL>",
                           "pair": "voxel_kernel<VOX_BRICK4_PAIR,CACHE_CELL>",
                           "quad": "voxel_kernel<VOX_BRICK4_QUAD,CACHE_CELL>",
+                          "quad_slide": "voxel_kernel<VOX_BRICK4_QUAD,CACHE_SLIDE>",
                           }.get(layout, "voxel_lds_kernel")
     else:
         roof["kernel"] = "march_kernel<%s,%s>" % (

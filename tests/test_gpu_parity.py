@@ -239,7 +239,8 @@ def blob256():
     return W.blob_grid(256)
 
 
-VOXEL_LAYOUTS = ["lds", "brick4", "linear", "cached", "linear_cached", "pair", "quad"]
+VOXEL_LAYOUTS = ["lds", "brick4", "linear", "cached", "linear_cached", "pair", "quad",
+                 "quad_slide"]
 
 
 @pytest.mark.parametrize("layout", VOXEL_LAYOUTS)
@@ -261,7 +262,7 @@ def test_voxel_layouts_bit_identical(ctx, blob256, monkeypatch):
     w = W.voxel(grid=blob256, n_views=6)
     sc = xrp_scene(None, w.deformation, voxels=blob256)
     imgs = {}
-    for layout in ["brick4", "linear", "cached", "linear_cached", "pair", "quad"]:
+    for layout in ["brick4", "linear", "cached", "linear_cached", "pair", "quad", "quad_slide"]:
         monkeypatch.setenv("XRP_VOXEL_LAYOUT", layout)
         imgs[layout], _ = X.render(ctx, sc, w.angles, w.resolution, w.ds,
                                    integrator="simple")
@@ -269,7 +270,7 @@ def test_voxel_layouts_bit_identical(ctx, blob256, monkeypatch):
         assert np.array_equal(img, imgs["brick4"]), layout
 
 
-@pytest.mark.parametrize("layout", ["brick4", "cached", "pair", "quad"])
+@pytest.mark.parametrize("layout", ["brick4", "cached", "pair", "quad", "quad_slide"])
 def test_voxel_boundary_ties(ctx, layout, monkeypatch):
     """Axis-aligned views put samples on the grid faces x = +-1 exactly;
     the per-ray FP64 range must count them as the reference does."""

@@ -423,14 +423,16 @@ bool compose_linear(const std::vector<xrp::HDeform>& defs, double M[9], double t
   return true;
 }
 
+// Brick-order copies of a voxel root that the voxel-root kernel reads.
+enum BrickSlot { SLOT_PLAIN = 0, SLOT_PAIR = 1, SLOT_QUAD = 2, N_SLOTS = 3 };
+
 // Device copies of a scene's voxel grids on one context: the linear copy of
 // every voxel node (preorder, matching the flattened node order), then, for
-// a voxel root, three slots for the brick-order copies of width 1, 2 and 4
-// that the voxel-root kernel reads.  A brick slot is filled on first use
-// (brick_width 1, 2 or 4; 0 asks for none), so a scene holds only the
-// copies its layouts need.
+// a voxel root, N_SLOTS brick-copy slots.  A brick slot is filled on first
+// use (slot < 0 asks for none), so a scene holds only the copies its
+// layouts need.
 int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs,
-                  int brick_width = 0) {
+                  int slot = -1) {
   std::lock_guard<std::mutex> g(g_registry_mu);
   const bool root_vox = sc->s->root.kind == xrp::K_VOXEL;
   auto put = [](const std::vector<float>& host, size_t bytes, void*& p) -> hipError_t {
@@ -462,19 +464,21 @@ int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs,
       }
       out.push_back(p);
     }
-    if (root_vox) out.insert(out.end(), 3, nullptr);
+    if (root_vox) out.insert(out.end(), N_SLOTS, nullptr);
     it = sc->uploads.emplace(ctx, std::move(out)).first;
     ctx->scenes.insert(sc);
   }
   std::vector<void*>& out = it->second;
-  if (root_vox && brick_width) {
-    const int slot = brick_width == 1 ? 0 : brick_width == 2 ? 1 : 2;
-    void*& p = out[out.size() - 3 + slot];
+  if (root_vox && slot >= 0) {
+    void*& p = out[out.size() - N_SLOTS + slot];
     if (!p) {
       const xrp::HNode& n = sc->s->root;
-      std::vector<float> br = brick_width == 1
-                                  ? brick4(*n.vox, n.nx, n.ny, n.nz)
-                                  : brick4_wide(*n.vox, n.nx, n.ny, n.nz, brick_width);
+      std::vector<float> br;
+      if (slot == SLOT_PLAIN) {
+        br = brick4(*n.vox, n.nx, n.ny, n.nz);
+      } else {
+        br = brick4_wide(*n.vox, n.nx, n.ny, n.nz, slot == SLOT_PAIR ? 2 : 4);
+      }
       hipError_t e = put(br, br.size() * sizeof(float), p);
       if (e != hipSuccess)
         return fail(XRP_ERR_ALLOC, std::string("voxel upload failed: ") +
@@ -500,7 +504,7 @@ struct LaunchPlan {
   std::vector<double> table;
   int precision = XRP_PRECISION_GUARDED;
   bool vox_kernel = false;
-  int layout = xrp::VOX_BRICK4_QUAD;
+  int layout = xrp::VOX_BRICK4_QUAD_SLIDE;
   double delta = 0.0;
 };
 
@@ -613,11 +617,12 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   mp.block = s.nodes.size() >= 24 ? 4 : 2;
   mp.n_views = (int)job.views.size();
   // 4^3 bricks of corner quads (two 16-byte loads per cell) with the
-  // per-lane cell cache measured fastest on the 256^3 voxel workload (9.85k
-  // 512^2 projections/s; plain bricks 7.94k, corner pairs 7.92k, uncached
-  // bricks 5.4k, cached linear 4.8k, LDS-staged 2.2k, linear 1.8k); the
-  // others stay selectable through XRP_VOXEL_LAYOUT for comparison.
-  int layout = VOX_BRICK4_QUAD;
+  // per-lane cell cache sliding along x measured fastest on the 256^3 voxel
+  // workload (11.6k 512^2 projections/s; quads with a plain cell cache
+  // 9.87k, plain bricks 7.94k, corner pairs 7.92k, uncached bricks 5.4k,
+  // cached linear 4.8k, LDS-staged 2.2k, linear 1.8k); the others stay
+  // selectable through XRP_VOXEL_LAYOUT for comparison.
+  int layout = VOX_BRICK4_QUAD_SLIDE;
   if (vox_kernel) {
     const char* env = std::getenv("XRP_VOXEL_LAYOUT");
     if (env) {
@@ -629,17 +634,18 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       if (e == "linear_cached") layout = VOX_LINEAR_CACHED;
       if (e == "pair") layout = VOX_BRICK4_PAIR;
       if (e == "quad") layout = VOX_BRICK4_QUAD;
+      if (e == "quad_slide") layout = VOX_BRICK4_QUAD_SLIDE;
     }
     const bool lin = voxel_layout_linear(layout);
     const size_t nlin = (size_t)s.root.nx * s.root.ny * s.root.nz;
-    const int width = layout == VOX_BRICK4_PAIR ? 2 : layout == VOX_BRICK4_QUAD ? 4 : 1;
+    const int width = layout == VOX_BRICK4_PAIR ? 2 : layout >= VOX_BRICK4_QUAD ? 4 : 1;
+    const int slot = width == 1 ? SLOT_PLAIN : width == 2 ? SLOT_PAIR : SLOT_QUAD;
     if (!lin) {
-      int rc = upload_voxels(ctx, sc, vptr, width);
+      int rc = upload_voxels(ctx, sc, vptr, slot);
       if (rc) return rc;
     }
     const size_t nb = vptr.size();
-    mp.vgrid = (const float*)(lin ? vptr.front()
-                                  : vptr[nb - 3 + (width == 1 ? 0 : width == 2 ? 1 : 2)]);
+    mp.vgrid = (const float*)(lin ? vptr.front() : vptr[nb - N_SLOTS + slot]);
     mp.vbytes = (uint32_t)(4ull * (lin ? nlin : (size_t)mp_bricks(s.root) * 64 * width));
     mp.vlin = (const float*)vptr.front();
     mp.vlin_bytes = (uint32_t)(4ull * nlin);

@@ -83,6 +83,8 @@ struct MarchParams {
 // VOX_BRICK4_PAIR / VOX_BRICK4_QUAD (cached): brick elements of 2 floats
 // (v[y], v[y+1]) or 4 floats (v[z][y], v[z+1][y], v[z][y+1], v[z+1][y+1]),
 // upper indices clamped, so a cell's corners come in 4 or 2 loads.
+// VOX_BRICK4_QUAD_SLIDE: the quad copy with a cell cache that, when a cell
+// moves by one along x, keeps the shared x plane and loads the other.
 enum VoxelLayout : int32_t {
   VOX_LINEAR = 0,
   VOX_BRICK4 = 1,
@@ -90,7 +92,8 @@ enum VoxelLayout : int32_t {
   VOX_BRICK4_CACHED = 3,
   VOX_LINEAR_CACHED = 4,
   VOX_BRICK4_PAIR = 5,
-  VOX_BRICK4_QUAD = 6
+  VOX_BRICK4_QUAD = 6,
+  VOX_BRICK4_QUAD_SLIDE = 7
 };
 inline bool voxel_layout_linear(int layout) {
   return layout == VOX_LINEAR || layout == VOX_LINEAR_CACHED;
