@@ -193,8 +193,15 @@ def bench_batch(args) -> int:
         flops += n * batch_flops_per_sample(w.obj)
     b = X.Batch(ctx, items)
     print(f"batch: {len(ws)} scenes planned", file=sys.stderr, flush=True)
-    # the autotuner times 9 tile shapes on the whole batch: opt-in here
-    shape = b.tune(repeats=1) if args.tune else (1, 1, 2)
+    # The autotuner times 10 tile shapes on the whole batch (opt-in here);
+    # without it, the shape it picked on MI355X for this batch: 8x2 blocks of
+    # 8x8 rays per work item, 2 workgroups per CU (5.32 s per launch; 4x1:
+    # 5.57 s, 1x1: 8.06 s).
+    if args.tune:
+        shape = b.tune(repeats=1)
+    else:
+        shape = (8, 2, 2)
+        b.set_shape(*shape)
     for _ in range(max(0, args.warmup - 1)):
         b.run()
         print("batch: warm-up run done", file=sys.stderr, flush=True)

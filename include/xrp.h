@@ -220,7 +220,7 @@ typedef struct {
 } xrp_batch_item;
 
 /* Work item of the queue: wave_rows x wave_cols blocks of 8x8 pixels that
- * one wave marches in sequence (at most 4 blocks); groups_per_cu persistent
+ * one wave marches in sequence (at most 16 blocks); groups_per_cu persistent
  * 256-thread workgroups are launched per compute unit. */
 typedef struct {
   int32_t wave_rows;

@@ -78,7 +78,8 @@ def test_batch_shapes_and_replays(ctx, batch_setup):
     b.run()
     base = [b.read(k) for k in range(len(items))]
     sc0, a0, r0, ds0, kw0 = items[0]
-    for shape in [(2, 2, 2), (2, 1, 4), (1, 4, 1), (4, 1, 3), (1, 2, 8)]:
+    for shape in [(2, 2, 2), (2, 1, 4), (1, 4, 1), (4, 1, 3), (1, 2, 8), (8, 1, 2),
+                  (4, 4, 1), (16, 1, 2)]:
         b.set_shape(*shape)
         for rep in range(3):
             st = b.run()
@@ -91,7 +92,7 @@ def test_batch_shapes_and_replays(ctx, batch_setup):
 def test_batch_tune_and_latency(ctx, batch_setup):
     specs, scenes, items, b = batch_setup
     shape = b.tune(repeats=1)
-    assert 1 <= shape[0] * shape[1] <= 4 and shape[2] >= 1
+    assert 1 <= shape[0] * shape[1] <= 16 and shape[2] >= 1
     st = b.run()
     lat = b.latencies()
     assert np.all(lat > 0)
@@ -120,7 +121,7 @@ def test_batch_rejects_bad_input(ctx):
     b = X.Batch(ctx, [(sc, [(0.0, 90.0)], 16, 0.02)])
     try:
         with pytest.raises(X.XrpError):
-            b.set_shape(3, 2, 1)
+            b.set_shape(6, 3, 1)
         with pytest.raises(X.XrpError):
             b.output_ptr(5)  # the library's own range check
         with pytest.raises(IndexError):

@@ -189,6 +189,56 @@ def test_safe_radius_settling_changes_no_bits(ctx, tmp_path):
             assert np.array_equal(img, ref[name + "_" + integ]), (name, integ)
 
 
+_CULL_DEFORMATION = {"type": "affine",
+                     "matrix": [[1.2, 0.15, -0.1], [-0.2, 0.9, 0.25], [0.1, -0.15, 1.1]]}
+
+_NO_CULL_SCRIPT = r"""
+import sys, numpy as np
+import xray_projection_render_amd as X
+from xray_projection_render_amd import workloads as W
+ctx = X.Context(0)
+dfm = %r
+out = {}
+for seed, n, d in ((4, 80, None), (1, 200, None), (6, 120, dfm)):
+    obj, _, _ = W.random_scene(seed, max_prims=n)
+    sc = X.Scene.from_dict(obj, d)
+    ds = max(sc.auto_ds(), 0.004)
+    for integ in ("simple", "hierarchical"):
+        img, st = X.render(ctx, sc, [(17.0, 60.0), (250.0, 130.0)], 40, ds, integrator=integ)
+        out["%%d_%%s" %% (seed, integ)] = img
+    sc.close()
+ctx.close()
+np.savez(sys.argv[1], **out)
+"""
+
+
+def test_child_culling_changes_no_bits(ctx, tmp_path):
+    """Per-wave culling of the root collection's children (bounding spheres
+    against each ray's line in object space, also under an affine map)
+    leaves out only children that are zero along the wave's rays: audit
+    finds no FP32-clear sample that differs from FP64, and renders with
+    culling off (XRP_CULL=0, read once per process, hence a child process)
+    give the same bits."""
+    import subprocess
+    import sys as _sys
+    path = tmp_path / "nocull.npz"
+    env = dict(os.environ, XRP_CULL="0")
+    subprocess.run([_sys.executable, "-c", _NO_CULL_SCRIPT % (_CULL_DEFORMATION,), str(path)],
+                   env=env, check=True, timeout=300)
+    ref = np.load(path)
+    for seed, n, d in ((4, 80, None), (1, 200, None), (6, 120, _CULL_DEFORMATION)):
+        obj, _, _ = W.random_scene(seed, max_prims=n)
+        sc = xrp_scene(obj, d)
+        ds = max(sc.auto_ds(), 0.004)
+        for integ in ("simple", "hierarchical"):
+            angles = [(17.0, 60.0), (250.0, 130.0)]
+            img, st = X.render(ctx, sc, angles, 40, ds, integrator=integ, audit=True)
+            assert st["audit_mismatch"] == 0, (seed, integ)
+            assert np.array_equal(img, ref["%d_%s" % (seed, integ)]), (seed, integ)
+        err, _, _, _ = _compare(ctx, obj, [(17.0, 60.0)], 40, ds, deformation=d)
+        assert err <= TOL
+
+
 @pytest.mark.parametrize("name", ["csg", "lattice"])
 def test_safe_radius_skipping_hierarchical(ctx, name):
     """Coarse samples of the hierarchical march inside a safe radius are

@@ -46,8 +46,11 @@ struct alignas(16) DevNode {
   float f[16];
   float g[8];
   double d[14];
-  const float* vox;  // VOXEL: device pointer, [z][x][y] layout
-  double pad_;
+  union {
+    const float* vox;  // VOXEL: device pointer, [z][x][y] layout
+    float bsph[4];     // SPHERE/BOX/PPED/CYL: object-space bounding sphere
+                       // (centre, radius), enclosing the primitive's closure
+  };
 };
 static_assert(sizeof(DevNode) == 256, "DevNode must stay 256 bytes");
 

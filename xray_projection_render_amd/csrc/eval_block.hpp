@@ -225,10 +225,21 @@ __device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
   }
 }
 
+// The children of the root collection that can be non-zero somewhere along
+// the rays of the calling wave, as increasing node indices in LDS; n < 0
+// walks every child.  A child left out has density 0 at every sample of
+// those rays, so its terms (+0) and its safe-radius bound (the radius is
+// only used along the same rays) change nothing.
+struct Cull {
+  const int16_t* idx = nullptr;
+  int n = -1;
+};
+
 template <int D, int S, bool DIST>
 __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
-                                         unsigned& amb_out, float& dist);
+                                         unsigned& amb_out, float& dist,
+                                         const Cull& cl = Cull());
 
 // Tessellation wrap (objects.go:568-582) per sample, then the unit cell's
 // collection for the samples that passed the bounds and seam guards.
@@ -378,14 +389,20 @@ __device__ __forceinline__ void leaf_blk(const DevNode* __restrict__ N, int c,
 template <int D, int S, bool DIST>
 __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
-                                         unsigned& amb_out, float& dist) {
+                                         unsigned& amb_out, float& dist, const Cull& cl) {
   const DevNode& n = N[i];
   const bool greedy = n.greedy;
   // out[s] holds the running sum until sample s finishes
 #pragma unroll
   for (int s = 0; s < S; ++s) out[s] = 0.0;
   unsigned done = 0, amb = 0, hit = 0;
-  for (int c = n.first; c < n.next; c = N[c].next) {
+  const bool listed = cl.n >= 0;
+  int t = 0;
+  int c = !listed ? n.first
+                  : (cl.n > 0 ? __builtin_amdgcn_readfirstlane((int)cl.idx[0]) : n.next);
+  for (; c < n.next;
+       c = !listed ? N[c].next
+                   : (++t < cl.n ? __builtin_amdgcn_readfirstlane((int)cl.idx[t]) : n.next)) {
     const unsigned act = live & ~done;
     if (!act) break;
     double v[S];
@@ -421,21 +438,23 @@ __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
 
 // dist (DIST only) starts at +inf in the caller and is lowered by every
 // node the last sample visits.
+// cl: the wave's culled child list when the root is a collection.
 template <int S, bool DIST = false>
 __device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PBlock<S>& p,
                                          unsigned live, double (&out)[S], unsigned& amb,
-                                         float& dist) {
+                                         float& dist, const Cull& cl = Cull()) {
   if (N[0].kind == K_COLL)
-    coll_blk<0, S, DIST>(N, 0, p, live, out, amb, dist);
+    coll_blk<0, S, DIST>(N, 0, p, live, out, amb, dist, cl);
   else
     leaf_blk<0, S, DIST>(N, 0, p, live, out, amb, dist);
   amb &= live;
 }
 template <int S>
 __device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PBlock<S>& p,
-                                         unsigned live, double (&out)[S], unsigned& amb) {
+                                         unsigned live, double (&out)[S], unsigned& amb,
+                                         const Cull& cl = Cull()) {
   float unused = 0.0f;
-  root_blk<S, false>(N, p, live, out, amb, unused);
+  root_blk<S, false>(N, p, live, out, amb, unused, cl);
 }
 
 template <int S>

@@ -51,8 +51,9 @@ struct MarchParams {
   int32_t audit;           // evaluate every sample in FP64 as well
   int32_t view_offset;     // first view index of this launch
   int32_t out_view_stride; // floats between consecutive views in out
-  // Voxel-root kernel: world -> object affine map (the composed linear
-  // deformation chain), grid dims, brick layout and the grid-space guard.
+  // Voxel-root kernel (and culled collections: M, t): world -> object
+  // affine map (the composed linear deformation chain), grid dims, brick
+  // layout and the grid-space guard.
   const float* vgrid;      // linear [z][x][y], or 4^3 bricks of 1, 2 or 4 floats
   int32_t vn[3];           // nx ny nz
   int32_t vbricks[3];      // bricks per axis (x, y, z)
@@ -74,7 +75,9 @@ struct MarchParams {
   float skip_inv;
   // Guarded simple march, per-scene kernel: samples per node visit (2 or 4).
   int32_t block;
-  int32_t pad4_;
+  // Guarded marches: cull the root collection's children per wave
+  // (march_ray.hpp wave_cull); M and t then hold the composed deformation.
+  int32_t cull;
 };
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with

@@ -47,6 +47,64 @@ __device__ __forceinline__ void clip_range(const MarchParams& p, double t_in, do
   }
 }
 
+// Most children a culled root collection may have (p.cull is off above).
+constexpr int kMaxCull = 256;
+
+// The wave's culled child list (Cull, eval_block.hpp) when p.cull is set:
+// the root is a collection of at most kMaxCull children and the
+// deformations compose to x -> M x + t (p.M, p.t), so each ray is a line in
+// object space.  A bounded child (sphere, box, parallelepiped, cylinder) is
+// kept when its bounding sphere, which the host inflates past rounding,
+// meets the line of any lane's ray; every other child is kept.  The test is
+// FP64 on the infinite line, a superset of the samples marched.  Called by
+// every lane of the wave.
+template <int QW>
+__device__ __forceinline__ Cull wave_cull(const MarchParams& p, int v, int i, int j,
+                                          bool valid) {
+  __shared__ int16_t lists[QW][kMaxCull];
+  Cull cl;
+  if (!p.cull) return cl;
+  int16_t* L = lists[QW == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
+  bool has = false;
+  double o[3] = {0.0, 0.0, 0.0}, d[3] = {0.0, 0.0, 0.0};
+  if (valid) {
+    const Ray r = make_ray(p, p.views[p.view_offset + v], i, j);
+    double t_in, t_out;
+    if (clip_ray(p, r, t_in, t_out)) {
+      has = true;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        o[c] = p.M[3 * c] * r.o[0] + p.M[3 * c + 1] * r.o[1] + p.M[3 * c + 2] * r.o[2] + p.t[c];
+        d[c] = p.M[3 * c] * r.d[0] + p.M[3 * c + 1] * r.d[1] + p.M[3 * c + 2] * r.d[2];
+      }
+    }
+  }
+  const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const double inv = dd > 0.0 ? 1.0 / dd : 0.0;
+  const DevNode* __restrict__ N = p.nodes;
+  int n = 0;
+  for (int c = N[0].first; c < N[0].next; c = N[c].next) {
+    const DevNode& ch = N[c];
+    bool keep = has;
+    if (has && (ch.kind == K_SPHERE || ch.kind == K_BOX || ch.kind == K_PPED ||
+                ch.kind == K_CYL)) {
+      const double w0 = (double)ch.bsph[0] - o[0], w1 = (double)ch.bsph[1] - o[1],
+                   w2 = (double)ch.bsph[2] - o[2];
+      const double t = (w0 * d[0] + w1 * d[1] + w2 * d[2]) * inv;
+      const double q0 = w0 - t * d[0], q1 = w1 - t * d[1], q2 = w2 - t * d[2];
+      const double rr = (double)ch.bsph[3];
+      keep = q0 * q0 + q1 * q1 + q2 * q2 <= rr * rr;
+    }
+    if (__ballot(keep)) {
+      if ((threadIdx.x & 63) == 0) L[n] = (int16_t)c;
+      ++n;
+    }
+  }
+  cl.idx = L;
+  cl.n = n;
+  return cl;
+}
+
 // Optical depth T (flat field included) of pixel (i, j) of view v (v counts
 // from p.view_offset), simple integrator, FP64 evaluator.
 __device__ __forceinline__ double march_T_simple64(const MarchParams& p, int v, int i, int j,
@@ -80,7 +138,7 @@ __device__ __forceinline__ double march_T_simple64(const MarchParams& p, int v, 
 // and so the same bits, as the queued form below.
 template <int B>
 __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, int v, int i,
-                                                         int j, RayCounts& rc) {
+                                                         int j, RayCounts& rc, const Cull& cl) {
   // per-ray counts fit 32 bits (a ray has fewer than 2^31 samples)
   unsigned n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
   const double ds = p.ds, dm = p.dm;
@@ -121,7 +179,7 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
       double vv[B];
       unsigned amb = 0;
       float dist = INFINITY;
-      root_blk<B, true>(p.nodes, P, live, vv, amb, dist);
+      root_blk<B, true>(p.nodes, P, live, vv, amb, dist, cl);
       const bool last_amb = (amb >> (B - 1)) & 1u;
       // FP64 re-evaluation (under audit of every sample; the FP32-clear
       // ones must agree) through one call site
@@ -212,7 +270,7 @@ struct AmbQueue {
 // audit the re-evaluations are made inline.
 template <int B, int QW>
 __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, int i, int j,
-                                                  bool valid, RayCounts& rc) {
+                                                  bool valid, RayCounts& rc, const Cull& cl) {
   __shared__ AmbQueue queues[QW];
   AmbQueue& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
   const int lane = threadIdx.x & 63;
@@ -267,7 +325,7 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
       double vv[B];
       unsigned amb = 0;
       float dist = INFINITY;
-      root_blk<B, true>(p.nodes, P, live, vv, amb, dist);
+      root_blk<B, true>(p.nodes, P, live, vv, amb, dist, cl);
       n_reeval += (unsigned)__builtin_popcount(amb);
       if (p.audit) {
         // under audit every sample in FP64, inline (the FP32-clear ones
@@ -419,8 +477,8 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
                                            const double (&d)[3], const double (&s)[B],
                                            unsigned live, double (&rho)[B],
                                            unsigned long long& n_reeval,
-                                           unsigned long long& n_miss, float* dist = nullptr,
-                                           bool* last_amb = nullptr) {
+                                           unsigned long long& n_miss, const Cull& cl,
+                                           float* dist = nullptr, bool* last_amb = nullptr) {
   if constexpr (PREC == PREC_FP64) {
 #pragma unroll
     for (int q = 0; q < B; ++q)
@@ -438,10 +496,10 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
     deform_blk<B>(p.defs, p.n_defs, P);
     unsigned amb = 0;
     if constexpr (DIST) {
-      root_blk<B, true>(p.nodes, P, live, rho, amb, *dist);
+      root_blk<B, true>(p.nodes, P, live, rho, amb, *dist, cl);
       *last_amb = (amb >> (B - 1)) & 1u;
     } else {
-      root_blk<B>(p.nodes, P, live, rho, amb);
+      root_blk<B>(p.nodes, P, live, rho, amb, cl);
     }
     unsigned redo = p.audit ? live : amb;
     while (redo) {
@@ -471,7 +529,7 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
 // false and still help drain the queue); T is meaningful for valid lanes.
 template <int PREC, int QW>
 __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int i, int j,
-                                               bool valid, RayCounts& rc) {
+                                               bool valid, RayCounts& rc, const Cull& cl) {
   constexpr int B = PREC == PREC_FP64 ? 1 : kHierBlock;
   __shared__ HierQueue queues[QW];
   HierQueue& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
@@ -519,7 +577,7 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
       constexpr bool kDist = PREC == PREC_GUARDED;
       float dist = INFINITY;
       bool last_amb = true;
-      eval_along<PREC, B, kDist>(p, r.o, r.d, s, live, rho, n_reeval, n_miss, &dist,
+      eval_along<PREC, B, kDist>(p, r.o, r.d, s, live, rho, n_reeval, n_miss, cl, &dist,
                                  &last_amb);
 #pragma unroll
       for (int q = 0; q < B; ++q) {
@@ -613,7 +671,7 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
             constexpr bool kDist = PREC == PREC_GUARDED;
             float dist = INFINITY;
             bool last_amb = true;
-            eval_along<PREC, B, kDist>(p, o, d, s, live, rf, n_reeval, n_miss, &dist,
+            eval_along<PREC, B, kDist>(p, o, d, s, live, rf, n_reeval, n_miss, cl, &dist,
                                        &last_amb);
 #pragma unroll
             for (int q = 0; q < B; ++q)
@@ -670,14 +728,16 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
 template <int PREC, int INTEG, int QW, int B = kBlock, bool DEFER = false>
 __device__ __forceinline__ double march_T(const MarchParams& p, int v, int i, int j, bool valid,
                                           RayCounts& rc) {
+  Cull cl;
+  if constexpr (PREC == PREC_GUARDED) cl = wave_cull<QW>(p, v, i, j, valid);
   if constexpr (INTEG == INT_HIERARCHICAL) {
-    return march_T_hier<PREC, QW>(p, v, i, j, valid, rc);
+    return march_T_hier<PREC, QW>(p, v, i, j, valid, rc, cl);
   } else if constexpr (PREC == PREC_FP64) {
     return valid ? march_T_simple64(p, v, i, j, rc) : 0.0;
   } else if constexpr (DEFER) {
-    return march_T_guarded<B, QW>(p, v, i, j, valid, rc);
+    return march_T_guarded<B, QW>(p, v, i, j, valid, rc, cl);
   } else {
-    return valid ? march_T_guarded_inline<B>(p, v, i, j, rc) : 0.0;
+    return valid ? march_T_guarded_inline<B>(p, v, i, j, rc, cl) : 0.0;
   }
 }
 

@@ -652,6 +652,62 @@ double min_feature(const HNode& n) {
   return INFINITY;
 }
 
+// Object-space bounding sphere of a bounded primitive, stored in FP32 with
+// the radius inflated past the rounding of the stored centre and radius, so
+// the sphere encloses the primitive's closure (per-wave culling of the root
+// collection's children tests it against each ray's line).
+void bounding_sphere(const HNode& n, DevNode& dn) {
+  double c[3] = {0, 0, 0}, r = 0.0;
+  switch (n.kind) {
+    case K_SPHERE:
+      for (int k = 0; k < 3; ++k) c[k] = n.c[k];
+      r = std::fabs(n.a);
+      break;
+    case K_BOX:
+      for (int k = 0; k < 3; ++k) {
+        c[k] = n.c[k];
+        r += 0.25 * n.v[k] * n.v[k];
+      }
+      r = std::sqrt(r);
+      break;
+    case K_PPED:
+      for (int k = 0; k < 3; ++k)
+        c[k] = n.c[k] + 0.5 * (n.pv[k] + n.pv[3 + k] + n.pv[6 + k]);
+      for (int m = 0; m < 8; ++m) {
+        double e = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          double q = n.c[k] + ((m & 1) ? n.pv[k] : 0.0) + ((m & 2) ? n.pv[3 + k] : 0.0) +
+                     ((m & 4) ? n.pv[6 + k] : 0.0) - c[k];
+          e += q * q;
+        }
+        r = std::max(r, std::sqrt(e));
+      }
+      break;
+    case K_CYL: {
+      double h = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        c[k] = 0.5 * (n.c[k] + n.v[k]);
+        h += 0.25 * (n.v[k] - n.c[k]) * (n.v[k] - n.c[k]);
+      }
+      r = std::sqrt(h + n.a * n.a);
+      break;
+    }
+    default:
+      return;
+  }
+  double m = std::fabs(r);
+  for (int k = 0; k < 3; ++k) {
+    dn.bsph[k] = (float)c[k];
+    m = std::max(m, std::fabs(c[k]));
+  }
+  // FP32 rounding of centre and radius is <= 2^-24 of m each; the relative
+  // and absolute terms also cover the host's FP64 rounding
+  const double rr = r * (1.0 + 1e-6) + 4e-7 * m + 1e-12;
+  dn.bsph[3] = (float)rr;
+  if ((double)dn.bsph[3] < rr) dn.bsph[3] = std::nextafter(dn.bsph[3], INFINITY);
+  if (!std::isfinite(dn.bsph[3])) dn.bsph[3] = INFINITY;
+}
+
 void flatten(const HNode& n, int depth, std::vector<DevNode>& out, Scene& s) {
   int idx = (int)out.size();
   out.emplace_back();
@@ -760,6 +816,7 @@ void flatten(const HNode& n, int depth, std::vector<DevNode>& out, Scene& s) {
       dn.greedy = n.greedy ? 1 : 0;
       break;
   }
+  bounding_sphere(n, dn);
   if (n.kind == K_TESS) {
     dn.first = (int)out.size();
     flatten(n.kids[0], depth + 1, out, s);
