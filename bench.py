@@ -358,7 +358,8 @@ def bench_surrogate(args) -> int:
                            "fraction_above_tolerance": frac_over, "tolerance": tol},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
-                     "traffic": measured_traffic("sur_render_kernel", "surrogate"),
+                     "traffic": measured_traffic("sur_render_kernel",
+                                                 f"surrogate {n_views} views"),
                      "kernel": "sur_render_kernel", "kernel_ms": pure["surrogate_ms"],
                      "per_sample": f"{SURROGATE_FLOPS_PER_SAMPLE:.0f} flop (MLP)",
                      "samples_per_launch": sur_samples},
