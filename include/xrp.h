@@ -351,6 +351,11 @@ int xrp_diag_mfma_bf16(xrp_context *ctx, const float *a, const float *b, float *
  * per XCD through its L2 with 16-byte loads; the second ceiling the voxel
  * kernel's gathers are compared with. */
 int xrp_diag_l2_bandwidth(xrp_context *ctx, double *gbps);
+/* Diagnostic, no counterpart in the reference: the largest absolute errors
+ * of the hardware sine and cosine (argument in revolutions) over every FP32
+ * argument in [-0.5, 0.5], against FP64; the FP32 gyroid evaluators' guard
+ * bands assume at most 1.3e-7. */
+int xrp_diag_sincos_error(xrp_context *ctx, double *max_sin, double *max_cos);
 
 /* ---- reference JSON API (api.go:82-180) ----------------------------------- */
 /* Renders as the reference's RenderProjections does (PNG frames,

@@ -13,3 +13,14 @@ def test_l2_read_bandwidth_is_plausible(ctx):
     gbps = X.diag_l2_bandwidth(ctx)
     print(f"L2 read bandwidth {gbps:.0f} GB/s", flush=True)
     assert 4000.0 < gbps < 79000.0
+
+
+def test_hardware_sincos_error_within_guard_assumption(ctx):
+    """The FP32 gyroid evaluators use the hardware sine/cosine on arguments
+    reduced to [-0.5, 0.5] revolutions and size their guard bands for an
+    absolute error of at most 1.3e-7 there; the library re-measures it over
+    every FP32 argument in that range against FP64."""
+    es, ec = X.diag_sincos_error(ctx)
+    print(f"hardware sin/cos max abs error {es:.4e} / {ec:.4e}", flush=True)
+    assert 0.0 < es <= 1.3e-7
+    assert 0.0 < ec <= 1.3e-7

@@ -202,6 +202,7 @@ def lib() -> ctypes.CDLL:
         "xrp_surrogate_destroy": (None, [P]),
         "xrp_diag_mfma_bf16": (I, [P, FP, FP, FP]),
         "xrp_diag_l2_bandwidth": (I, [P, DP]),
+        "xrp_diag_sincos_error": (I, [P, DP, DP]),
         "RenderProjections": (ctypes.POINTER(ctypes.c_char), [ctypes.c_char_p]),
         "FreeString": (None, [ctypes.POINTER(ctypes.c_char)]),
     }
@@ -617,6 +618,14 @@ class Surrogate:
             self.close()
         except Exception:
             pass
+
+
+def diag_sincos_error(ctx: Context) -> Tuple[float, float]:
+    """Largest absolute errors of the hardware sine and cosine over every FP32
+    argument in [-0.5, 0.5] revolutions (xrp_diag_sincos_error)."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    check(lib().xrp_diag_sincos_error(ctx.ptr, ctypes.byref(a), ctypes.byref(b)))
+    return float(a.value), float(b.value)
 
 
 def diag_l2_bandwidth(ctx: Context) -> float:

@@ -1918,6 +1918,31 @@ int xrp_diag_mfma_bf16(xrp_context* ctx, const float* a, const float* b, float* 
   return XRP_OK;
 }
 
+int xrp_diag_sincos_error(xrp_context* ctx, double* max_sin, double* max_cos) {
+  if (!ctx || !max_sin || !max_cos) return fail(XRP_ERR_NULL, "NULL argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  DevBuf out;
+  HIPCHK(out.ensure(2 * sizeof(unsigned)));
+  hipError_t he = hipMemsetAsync(out.p, 0, 2 * sizeof(unsigned), ctx->stream);
+  if (he == hipSuccess) {
+    int e = xrp::launch_sincos_error((unsigned*)out.p, ctx->stream);
+    if (e) he = (hipError_t)e;
+  }
+  unsigned h[2] = {0, 0};
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(h, out.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+  out.release();
+  if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+  float fs, fc;
+  std::memcpy(&fs, &h[0], sizeof fs);
+  std::memcpy(&fc, &h[1], sizeof fc);
+  *max_sin = fs;
+  *max_cos = fc;
+  return XRP_OK;
+}
+
 int xrp_diag_l2_bandwidth(xrp_context* ctx, double* gbps) {
   if (!ctx || !gbps) return fail(XRP_ERR_NULL, "NULL argument");
   std::lock_guard<std::mutex> lk(ctx->mu);

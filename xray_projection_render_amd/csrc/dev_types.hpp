@@ -32,6 +32,14 @@ enum DefKind : int32_t {
 constexpr int kMaxDepth = 3;
 constexpr int kMaxDeform = 8;
 
+// Absolute error bound of the hardware sine and cosine (v_sin_f32 and
+// v_cos_f32, argument in revolutions) over every FP32 argument in
+// [-0.5, 0.5]: the largest error against FP64 sin/cos(2 pi t) over all of
+// them is 1.254e-7 on MI355X (xrp_diag_sincos_error re-measures it).  The
+// FP32 gyroid evaluators reduce their argument to that range exactly and
+// the guard bands carry this bound.
+constexpr double kHwSinCosErr = 1.3e-7;
+
 // 256-byte node.  f[] holds FP32 copies for the guarded march, d[] the
 // FP64 values the exact evaluator uses (same values the reference uses).
 // g[] holds the FP32 guard bands, recomputed on the host for every render
@@ -59,7 +67,8 @@ static_assert(sizeof(DevNode) == 256, "DevNode must stay 256 bytes");
 //  BOX    f: cx cy cz hx hy hz rho  d: cx cy cz hx hy hz rho    (h = 0.5*side)
 //  PPED   f: ox oy oz m00..m22 rho  d: ox oy oz m[9] rho  (m = M^-1 column-major)
 //  CYL    f: p0(3) v(3) vv inv_vv r2 rho   d: p0(3) v(3) vv r rho
-//  GYROID f: cx cy cz inv_scale thick rho  d: cx cy cz scale thick rho
+//  GYROID f: cx cy cz inv_scale thick rho inv_scale/(2 pi)
+//         d: cx cy cz scale thick rho
 //  TESS   f: bounds(6) ucmin(3) dx dy dz inv_dx inv_dy inv_dz
 //         d: bounds(6) ucmin(3) ucmax(3)   (dx = ucmax - ucmin, objects.go:574)
 //  VOXEL  f: (nx-1)/2 (ny-1)/2 (nz-1)/2

@@ -64,12 +64,10 @@ __device__ __forceinline__ float cyl_dist(const DevNode& n, float x, float y, fl
   return fabsf(fmaxf(ra, aa)) - n.g[5];
 }
 __device__ __forceinline__ float gyroid_dist(const DevNode& n, float x, float y, float z) {
-  const float qx = (x - n.f[0]) * n.f[3], qy = (y - n.f[1]) * n.f[3],
-              qz = (z - n.f[2]) * n.f[3];
   float sx, cxs, sy, cys, sz, czs;
-  sincosf(qx, &sx, &cxs);
-  sincosf(qy, &sy, &cys);
-  sincosf(qz, &sz, &czs);
+  sincos_rev((x - n.f[0]) * n.f[6], sx, cxs);
+  sincos_rev((y - n.f[1]) * n.f[6], sy, cys);
+  sincos_rev((z - n.f[2]) * n.f[6], sz, czs);
   const float a = fabsf(fmaf(sx, cys, fmaf(sy, czs, sz * cxs))) - n.f[4];
   return (fabsf(a) - n.g[0]) * n.g[4];
 }
@@ -204,15 +202,14 @@ __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
 template <int S, bool DIST>
 __device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
                                            double (&v)[S], unsigned& amb, float& dist) {
-  const float cx = n.f[0], cy = n.f[1], cz = n.f[2], sc = n.f[3], th = n.f[4], g = n.g[0];
+  const float cx = n.f[0], cy = n.f[1], cz = n.f[2], rv = n.f[6], th = n.f[4], g = n.g[0];
   const double rho = n.d[5];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    float qx = (p.x[s] - cx) * sc, qy = (p.y[s] - cy) * sc, qz = (p.z[s] - cz) * sc;
     float sx, cxs, sy, cys, sz, czs;
-    sincosf(qx, &sx, &cxs);
-    sincosf(qy, &sy, &cys);
-    sincosf(qz, &sz, &czs);
+    sincos_rev((p.x[s] - cx) * rv, sx, cxs);
+    sincos_rev((p.y[s] - cy) * rv, sy, cys);
+    sincos_rev((p.z[s] - cz) * rv, sz, czs);
     float gg = fmaf(sx, cys, fmaf(sy, czs, sz * cxs));
     float a = fabsf(gg) - th;
     v[s] = a < 0.0f ? rho : 0.0;

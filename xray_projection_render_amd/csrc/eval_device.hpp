@@ -319,15 +319,24 @@ __device__ __forceinline__ GR cyl32(const DevNode& n, float x, float y,
   return {in ? n.d[8] : 0.0, !(in || out)};
 }
 
+// sin(2 pi t) and cos(2 pi t) on the hardware unit: t is reduced exactly to
+// [-0.5, 0.5] (t - rint(t) has no rounding error), where the error of
+// v_sin_f32 / v_cos_f32 is at most kHwSinCosErr.
+__device__ __forceinline__ void sincos_rev(float t, float& s, float& c) {
+  const float r = t - rintf(t);
+  s = __builtin_amdgcn_sinf(r);
+  c = __builtin_amdgcn_cosf(r);
+}
+
+// objects.go:1014-1032 in FP32: the arguments are taken in revolutions,
+// t = (p - c) / (2 pi scale), the product rounding like that of the
+// radians q = (p - c) / scale it replaces.
 __device__ __forceinline__ GR gyroid32(const DevNode& n, float x, float y,
                                        float z) {
-  float qx = (x - n.f[0]) * n.f[3];
-  float qy = (y - n.f[1]) * n.f[3];
-  float qz = (z - n.f[2]) * n.f[3];
   float sx, cx, sy, cy, sz, cz;
-  sincosf(qx, &sx, &cx);
-  sincosf(qy, &sy, &cy);
-  sincosf(qz, &sz, &cz);
+  sincos_rev((x - n.f[0]) * n.f[6], sx, cx);
+  sincos_rev((y - n.f[1]) * n.f[6], sy, cy);
+  sincos_rev((z - n.f[2]) * n.f[6], sz, cz);
   float g = fmaf(sx, cy, fmaf(sy, cz, sz * cx));
   float a = fabsf(g) - n.f[4];
   return {a < 0.0f ? n.d[5] : 0.0, fabsf(a) < n.g[0]};

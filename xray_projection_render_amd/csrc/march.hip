@@ -157,6 +157,34 @@ __global__ __launch_bounds__(256) void volume_quantize_kernel(const double* dens
   out[idx] = (unsigned char)(iq & 0xff);
 }
 
+// Largest |v_sin_f32(t) - sin(2 pi t)| and |v_cos_f32(t) - cos(2 pi t)| over
+// every FP32 t in [-0.5, 0.5] (both signs of each magnitude up to 0.5), the
+// FP64 functions as reference.  Each lane walks bit patterns lo + k * stride
+// up to hi; the maxima (non-negative floats, ordered like their bits) are
+// combined with atomicMax into out[0..1].
+__global__ __launch_bounds__(256) void sincos_error_kernel(unsigned hi, unsigned* out) {
+  const unsigned stride = gridDim.x * blockDim.x;
+  float es = 0.0f, ec = 0.0f;
+  for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b <= hi; b += stride) {
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const float t = __uint_as_float(b | (sgn ? 0x80000000u : 0u));
+      const double a = 6.283185307179586476925286766559 * (double)t;
+      es = fmaxf(es, (float)fabs((double)__builtin_amdgcn_sinf(t) - sin(a)));
+      ec = fmaxf(ec, (float)fabs((double)__builtin_amdgcn_cosf(t) - cos(a)));
+    }
+    if (b > hi - stride) break;  // the next b would pass hi (or wrap)
+  }
+  atomicMax(out + 0, __float_as_uint(es));
+  atomicMax(out + 1, __float_as_uint(ec));
+}
+
+int launch_sincos_error(unsigned* out, void* stream) {
+  hipLaunchKernelGGL(sincos_error_kernel, dim3(8192), dim3(256), 0, (hipStream_t)stream,
+                     0x3F000000u /* 0.5f */, out);
+  return (int)hipGetLastError();
+}
+
 int launch_volume(const MarchParams& p, int res, double* dens, unsigned long long* maxbits,
                   unsigned char* out, void* stream) {
   hipStream_t s = (hipStream_t)stream;

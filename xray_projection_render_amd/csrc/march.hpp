@@ -107,6 +107,10 @@ int launch_voxel(const MarchParams& p, int layout, void* stream);
 // words.
 size_t l2_probe_buffer_bytes();
 int launch_l2_probe(const void* buf, int workgroups, int iters, unsigned* out, void* stream);
+// Hardware sine/cosine error over all FP32 arguments in [-0.5, 0.5]
+// revolutions: out[0], out[1] (two uints, zeroed by the caller) receive the
+// bits of the largest absolute errors of sin and cos.
+int launch_sincos_error(unsigned* out, void* stream);
 // Exact march of a list: with vals, items are (view, i, j) pixels and
 // vals[k] receives the transmittance; without, items are (view, ti, tj)
 // 16x16 tiles rendered into p.out.

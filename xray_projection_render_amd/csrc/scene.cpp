@@ -782,6 +782,7 @@ void flatten(const HNode& n, int depth, std::vector<DevNode>& out, Scene& s) {
       dn.f[3] = (float)(1.0 / n.a);
       dn.f[4] = (float)n.b;
       dn.f[5] = (float)n.rho;
+      dn.f[6] = (float)(1.0 / (n.a * 6.283185307179586476925286766559));
       break;
     case K_VOXEL:
       dn.nx = n.nx;
@@ -1214,7 +1215,9 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         double cm = std::max({std::fabs(n.d[0]), std::fabs(n.d[1]), std::fabs(n.d[2])});
         double Q = (b + cm) / sc;
         double eq = (d + u * cm + 2 * u * (b + cm)) / sc + 2 * u * Q;
-        n.g[0] = (float)(kappa * (6 * eq + 64 * u));
+        // six sine/cosine factors, each off by the argument error eq and
+        // the hardware unit's kHwSinCosErr, plus the products and sums
+        n.g[0] = (float)(kappa * (6 * eq + 6 * kHwSinCosErr + 16 * u));
         // |grad g| of g = sin x cos y + sin y cos z + sin z cos x is at most
         // 1.77 (in q = p / scale units): |grad g|^2 peaks at 3 on a 721^3
         // grid of the 2 pi period, and between grid points it grows by at
