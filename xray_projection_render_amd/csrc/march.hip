@@ -33,11 +33,16 @@ namespace xrp {
 // faster than 16x16 tiles of 256 lanes).
 // Simple integrator: 4 waves per SIMD (<= 128 VGPRs), measured fastest on
 // the lattice and CSG workloads (+62 % / +12 % over the unconstrained 2
-// waves; 3 waves: -4 % / -8 %).  Hierarchical: 3 waves (168 VGPRs, the
-// toggle queue's state stays in registers), 7 % faster than 4 on the lattice.
+// waves; 3 waves: -4 % / -8 %).  Large collections (4 samples per node
+// visit, 24 nodes and up): the FP64 re-evaluations wait in the per-wave LDS
+// queue (march_ray.hpp) and the kernel runs 3 waves per SIMD (168 VGPRs;
+// at 4 the queue spills 352 B): 1.8-2.1x faster on 124-179-primitive
+// scenes than inline re-evaluation at 4 waves, the same bits.
+// Hierarchical: 3 waves (168 VGPRs, the toggle queue's state stays in
+// registers), 7 % faster than 4 on the lattice.
 template <int PREC, int INTEG, int B>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
-    INTEG == INT_HIERARCHICAL ? 3 : 4))) void march_kernel(
+    INTEG == INT_HIERARCHICAL ? 3 : (PREC == PREC_GUARDED && B == 4 ? 3 : 4)))) void march_kernel(
     MarchParams p) {
   const int lane = threadIdx.x;
   const int tiles_j = (p.res + 7) >> 3;
@@ -45,7 +50,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
   const int i = ti * 8 + (lane >> 3);
   const int j = tj * 8 + (lane & 7);
   RayCounts rc;
-  march_ray<PREC, INTEG, 1, B>(p, blockIdx.y, i, j, i < p.res && j < p.res, rc);
+  march_ray<PREC, INTEG, 1, B, PREC == PREC_GUARDED && INTEG == INT_SIMPLE && B == 4>(
+      p, blockIdx.y, i, j, i < p.res && j < p.res, rc);
   flush_counts(p.stats, rc);
 }
 
