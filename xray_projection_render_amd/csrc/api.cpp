@@ -1019,7 +1019,8 @@ struct xrp_batch {
     uint32_t n_tiles = 0;
   };
   std::vector<Group> groups;
-  DevBuf nodes, defs, views, tables, jobs, tile_start, counters, t_first, t_done, stats, out;
+  DevBuf nodes, defs, views, tables, jobs, tile_start, counters, t_first, t_done, stats, out,
+      sink;
   std::vector<size_t> out_off;        // floats
   xrp_tile_shape shape{1, 1, 2, 0};
   hipGraph_t graph = nullptr;
@@ -1127,6 +1128,7 @@ int apply_shape(xrp_batch* b) {
     bg.counter = (uint32_t*)b->counters.p + gi;
     bg.t_first = (unsigned long long*)b->t_first.p + g.first;
     bg.t_done = (unsigned long long*)b->t_done.p + g.first;
+    bg.sink = (unsigned long long*)b->sink.p;
     bg.n_jobs = (int)g.items.size();
     bg.n_tiles = g.n_tiles;
     bg.wave_rows = b->shape.wave_rows;
@@ -1296,6 +1298,7 @@ int xrp_batch_create(xrp_context* ctx, const xrp_batch_item* items, int n_items,
   HIPCHK(b->counters.ensure(b->groups.size() * sizeof(uint32_t)));
   HIPCHK(b->t_first.ensure((size_t)n_items * sizeof(unsigned long long)));
   HIPCHK(b->t_done.ensure((size_t)n_items * sizeof(unsigned long long)));
+  HIPCHK(b->sink.ensure(kBatchSinkBytes));
   HIPCHK(b->stats.ensure((size_t)n_items * ST_COUNT * sizeof(unsigned long long)));
   HIPCHK(b->out.ensure(out_floats * sizeof(float)));
   if (!all_nodes.empty())

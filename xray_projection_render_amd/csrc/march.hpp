@@ -126,11 +126,15 @@ struct BatchGroup {
   uint32_t* counter;            // work counter, zeroed by launch_batch_reset
   unsigned long long* t_first;  // per job: wall clock at its first tile start
   unsigned long long* t_done;   // per job: wall clock at its last tile end
+  unsigned long long* sink;     // kBatchSinkBytes: one 128-byte line per lane
   int32_t n_jobs;
   uint32_t n_tiles;
   int32_t wave_rows, wave_cols; // item: (8 wave_rows) x (8 wave_cols) pixels, one wave
   int32_t pad_;
 };
+// Lines that take the bookkeeping atomics of a wave's lanes other than its
+// first (batch.hip lane_slot).
+constexpr int kBatchSinkBytes = 64 * 128;
 int launch_batch(const BatchGroup& g, int precision, int integrator, int workgroups,
                  void* stream);
 // Zero the work counters and per-job stats, and reset the timestamps,
