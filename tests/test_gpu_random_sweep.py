@@ -1,6 +1,7 @@
 """GPU parity over seeded random scenes: collections of up to 200 mixed
 primitives (spheres, boxes, parallelepipeds, cylinders, gyroids) under
-seeded affine maps, random views, both integrators.  Every render runs with
+affine, strain and composed rigid + strain maps, random views, both
+integrators.  Every render runs with
 audit (each FP32-decided sample also evaluated in FP64; any difference is
 counted) and is compared with the FP64 oracle at the 1e-5 bar.  These
 scenes take the per-wave child culling, the queued FP64 re-evaluation of
@@ -23,10 +24,26 @@ def _affine(seed: int):
     return {"type": "affine", "matrix": m.tolist()}
 
 
+def _deformation(seed: int):
+    """Seeds cycle through no map, an affine map, a symmetric strain and a
+    composed rigid + strain chain: every chain the culling composes into
+    one affine map."""
+    k = seed % 4
+    if k == 1:
+        return _affine(seed)
+    if k == 2:
+        return {"type": "linear", "strains": [0.08, -0.05, 0.1, 0.04, -0.06, 0.03]}
+    if k == 3:
+        return {"type": "composed", "deformations": [
+            {"type": "rigid", "displacements": [0.05, -0.1, 0.02]},
+            {"type": "linear", "strains": [-0.1, 0.06, 0.0, 0.05, 0.02, -0.04]}]}
+    return None
+
+
 @pytest.mark.parametrize("seed", [11, 12, 13, 14, 15, 16])
 def test_random_scene_sweep(ctx, seed):
     obj, _, _ = W.random_scene(seed, max_prims=200)
-    dfm = _affine(seed) if seed % 2 else None
+    dfm = _deformation(seed)
     sc = xrp_scene(obj, dfm)
     ds = max(sc.auto_ds(), 0.004)
     rng = np.random.default_rng(seed)
