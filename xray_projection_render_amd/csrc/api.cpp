@@ -531,7 +531,7 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       allow_voxel_kernel && s.root.kind == K_VOXEL && compose_linear(s.defs, vM, vt) &&
       job.integrator == XRP_INTEGRATE_SIMPLE && !job.audit &&
       job.precision != XRP_PRECISION_FP64 &&
-      (size_t)mp_bricks(s.root) * 64 < ((size_t)1 << 28);
+      (size_t)mp_bricks(s.root) * 64 < ((size_t)1 << 30);
   int precision = job.precision;
   if (precision == XRP_PRECISION_AUTO)
     precision = (s.has_voxels && !vox_kernel) ? XRP_PRECISION_FP64 : XRP_PRECISION_GUARDED;
@@ -661,6 +661,13 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       if (e == "quad") layout = VOX_BRICK4_QUAD;
       if (e == "quad_slide") layout = VOX_BRICK4_QUAD_SLIDE;
     }
+    // The kernels address a copy with 32-bit byte offsets: a grid whose
+    // pair or quad copy would pass 4 GiB (2^28 voxels and up for quads)
+    // reads the plain brick copy with the cell cache instead.
+    const size_t belems = (size_t)mp_bricks(s.root) * 64;
+    if ((layout == VOX_BRICK4_PAIR && belems >= ((size_t)1 << 29)) ||
+        (layout >= VOX_BRICK4_QUAD && belems >= ((size_t)1 << 28)))
+      layout = VOX_BRICK4_CACHED;
     const bool lin = voxel_layout_linear(layout);
     const size_t nlin = (size_t)s.root.nx * s.root.ny * s.root.nz;
     const int width = layout == VOX_BRICK4_PAIR ? 2 : layout >= VOX_BRICK4_QUAD ? 4 : 1;
