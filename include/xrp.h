@@ -35,7 +35,7 @@ typedef enum {
   XRP_ERR_SCENE = 10,      /* object description rejected (reference FromMap rules) */
   XRP_ERR_DEFORMATION = 11,/* deformation description rejected */
   XRP_ERR_ARG = 12,        /* invalid parameter value (ds <= 0, unknown enum, ...) */
-  XRP_ERR_UNSUPPORTED = 13,/* valid for the reference but not for this build */
+  XRP_ERR_UNSUPPORTED = 13,/* valid for the reference, not supported by this library */
   XRP_ERR_IO = 14          /* file could not be read or written */
 } xrp_status;
 

@@ -2,7 +2,7 @@
 
 Each test restates one reference test (file:line) with the reference's
 inputs and tolerances; together they pin the oracle's integrators, objects
-and deformations before it is used to judge the HIP kernels.
+and deformations before it is used to check the HIP kernels.
 """
 import math
 
