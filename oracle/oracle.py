@@ -1,7 +1,7 @@
 """FP64 CPU oracle: ctypes wrapper and scene builder.
 
-TEST INFRASTRUCTURE ONLY.  The oracle is the checker for the HIP library
-(tests/, ``__graft_entry__.smoke()``) and the timed CPU baseline in
+TEST INFRASTRUCTURE ONLY.  The oracle is what the HIP library is compared
+with (tests/, ``__graft_entry__.smoke()``) and the timed CPU baseline in
 ``bench.py``.  The product never imports this module.
 
 ``build_object`` / ``build_deformation`` restate the reference's

@@ -1,8 +1,8 @@
 /*
  * xrp_oracle.c - FP64 CPU oracle for the per-pixel X-ray line integral.
  *
- * TEST INFRASTRUCTURE: this file is the checker and the timed CPU
- * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * TEST INFRASTRUCTURE: this file is what the tests compare the HIP library
+ * with, and the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline leg load it; the HIP library never does.
  *
  * It restates the reference renderer (igrega348/xray_projection_render)

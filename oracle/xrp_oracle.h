@@ -1,7 +1,7 @@
 /*
  * xrp_oracle.h - FP64 CPU restatement of the reference renderer's
  * per-pixel line-integral path.  TEST INFRASTRUCTURE ONLY: the oracle is
- * the checker for the HIP library and the timed CPU baseline; no product
+ * what the HIP library is compared with, and the timed CPU baseline; no product
  * code links, loads or calls it.
  *
  * Every function restates a reference function (file:line given beside
