@@ -194,13 +194,13 @@ def bench_batch(args) -> int:
     b = X.Batch(ctx, items)
     print(f"batch: {len(ws)} scenes planned", file=sys.stderr, flush=True)
     # The autotuner times 10 tile shapes on the whole batch (opt-in here);
-    # without it, the shape it picked on MI355X for this batch: 8x2 blocks of
-    # 8x8 rays per work item, 2 workgroups per CU (5.32 s per launch; 4x1:
-    # 5.57 s, 1x1: 8.06 s).
+    # without it, the fastest shape measured on MI355X for this batch: one
+    # 8x8 block of rays per work item, 2 workgroups per CU (2.99 s per
+    # launch; 2x2 blocks: 3.00 s, 4x1: 3.00 s, 8x2: 3.19 s).
     if args.tune:
         shape = b.tune(repeats=1)
     else:
-        shape = (8, 2, 2)
+        shape = (1, 1, 2)
         b.set_shape(*shape)
     for _ in range(max(0, args.warmup - 1)):
         b.run()

@@ -38,6 +38,28 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long lo
   atomicAdd(lane_slot(p, sink), first_lane() ? v : 0ull);
 }
 
+// Items a wave takes per counter operation.  A wave adds its stats and its
+// scene's last-end stamp once per run of consecutive items of one scene
+// (items are queued scene by scene), so a one-scene batch costs a quarter
+// of the same-address operations per item.
+constexpr uint32_t kItemsPerGrab = 4;
+
+// Close a wave's run of items of scene `job`: its stats, once, and its
+// last-end stamp.  All conditions are wave-uniform (job and p.stats are).
+__device__ __forceinline__ void close_run(const BatchGroup& g,
+                                          const MarchParams* __restrict__ jobs, int job,
+                                          const RayCounts& rc) {
+  const MarchParams& p = jobs[job];
+  if (p.stats) {
+    wave_add(p.stats + ST_SAMPLES, wave_sum(rc.samples), g.sink);
+    wave_add(p.stats + ST_REEVAL, wave_sum(rc.reeval), g.sink);
+    wave_add(p.stats + ST_AUDIT_MISS, wave_sum(rc.miss), g.sink);
+    wave_add(p.stats + ST_FINE, wave_sum(rc.fine), g.sink);
+    wave_add(p.stats + ST_SKIPPED, wave_sum(rc.skipped), g.sink);
+  }
+  atomicMax(lane_slot(g.t_done + job, g.sink), (unsigned long long)wall_clock64());
+}
+
 // Two waves per SIMD (simple integrator; the hierarchical form is left
 // unconstrained): the batched scenes' hot path (gyroids, up to 200
 // primitives) spills at 4 waves per SIMD and measured slower there, and
@@ -52,42 +74,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(INTEG == IN
   const int tile_i = 8 * g.wave_rows, tile_j = 8 * g.wave_cols;
   // Each wave takes items on its own: no LDS, no barriers.
   for (;;) {
-    const uint32_t got = atomicAdd(lane_slot(g.counter, g.sink), first_lane() ? 1u : 0u);
-    const uint32_t t = __builtin_amdgcn_readfirstlane(got);
-    if (t >= g.n_tiles) break;
-    // job holding item t: last job whose tile_start <= t
-    int lo = 0, hi = g.n_jobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (g.tile_start[mid] <= t)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const int job = __builtin_amdgcn_readfirstlane(lo);
-    const MarchParams& p = jobs[job];
-    const uint32_t local = t - g.tile_start[job];
-    const int tiles_i = (p.res + tile_i - 1) / tile_i, tiles_j = (p.res + tile_j - 1) / tile_j;
-    const uint32_t per_view = (uint32_t)tiles_i * (uint32_t)tiles_j;
-    const int v = (int)(local / per_view);
-    const uint32_t rem = local - (uint32_t)v * per_view;
-    const int ti = (int)(rem / (uint32_t)tiles_j), tj = (int)(rem - (uint32_t)ti * tiles_j);
-    atomicMin(lane_slot(g.t_first + job, g.sink), (unsigned long long)wall_clock64());
+    const uint32_t got =
+        atomicAdd(lane_slot(g.counter, g.sink), first_lane() ? kItemsPerGrab : 0u);
+    const uint32_t t0 = __builtin_amdgcn_readfirstlane(got);
+    if (t0 >= g.n_tiles) break;
+    const uint32_t t1 = min(t0 + kItemsPerGrab, g.n_tiles);
+    int run_job = -1;
     RayCounts rc;
-    for (int bi = 0; bi < g.wave_rows; ++bi)
-      for (int bj = 0; bj < g.wave_cols; ++bj) {
-        const int i = ti * tile_i + bi * 8 + (lane >> 3);
-        const int j = tj * tile_j + bj * 8 + (lane & 7);
-        march_ray<PREC, INTEG, 4, kBlock, true>(p, v, i, j, i < p.res && j < p.res, rc);
+    for (uint32_t t = t0; t < t1; ++t) {
+      // job holding item t: last job whose tile_start <= t
+      int lo = 0, hi = g.n_jobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g.tile_start[mid] <= t)
+          lo = mid;
+        else
+          hi = mid - 1;
       }
-    if (p.stats) {
-      wave_add(p.stats + ST_SAMPLES, wave_sum(rc.samples), g.sink);
-      wave_add(p.stats + ST_REEVAL, wave_sum(rc.reeval), g.sink);
-      wave_add(p.stats + ST_AUDIT_MISS, wave_sum(rc.miss), g.sink);
-      wave_add(p.stats + ST_FINE, wave_sum(rc.fine), g.sink);
-      wave_add(p.stats + ST_SKIPPED, wave_sum(rc.skipped), g.sink);
+      const int job = __builtin_amdgcn_readfirstlane(lo);
+      if (job != run_job) {
+        if (run_job >= 0) {
+          close_run(g, jobs, run_job, rc);
+          rc = RayCounts();
+        }
+        atomicMin(lane_slot(g.t_first + job, g.sink), (unsigned long long)wall_clock64());
+        run_job = job;
+      }
+      const MarchParams& p = jobs[job];
+      const uint32_t local = t - g.tile_start[job];
+      const int tiles_i = (p.res + tile_i - 1) / tile_i, tiles_j = (p.res + tile_j - 1) / tile_j;
+      const uint32_t per_view = (uint32_t)tiles_i * (uint32_t)tiles_j;
+      const int v = (int)(local / per_view);
+      const uint32_t rem = local - (uint32_t)v * per_view;
+      const int ti = (int)(rem / (uint32_t)tiles_j), tj = (int)(rem - (uint32_t)ti * tiles_j);
+      for (int bi = 0; bi < g.wave_rows; ++bi)
+        for (int bj = 0; bj < g.wave_cols; ++bj) {
+          const int i = ti * tile_i + bi * 8 + (lane >> 3);
+          const int j = tj * tile_j + bj * 8 + (lane & 7);
+          march_ray<PREC, INTEG, 4, kBlock, true>(p, v, i, j, i < p.res && j < p.res, rc);
+        }
     }
-    atomicMax(lane_slot(g.t_done + job, g.sink), (unsigned long long)wall_clock64());
+    close_run(g, jobs, run_job, rc);
   }
 }
 
