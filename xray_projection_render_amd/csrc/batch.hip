@@ -60,15 +60,14 @@ __device__ __forceinline__ void close_run(const BatchGroup& g,
   atomicMax(lane_slot(g.t_done + job, g.sink), (unsigned long long)wall_clock64());
 }
 
-// Two waves per SIMD (simple integrator; the hierarchical form is left
-// unconstrained): the batched scenes' hot path (gyroids, up to 200
+// Two waves per SIMD: the batched scenes' hot path (gyroids, up to 200
 // primitives) spills at 4 waves per SIMD and measured slower there, and
-// without the cap the child culling's state tipped the allocation past 256
-// registers (1 wave per SIMD).  With registers to spare the guarded march
-// queues its FP64 re-evaluations per wave (march_ray.hpp), 24 % faster on
-// the 32-scene batch.
+// without the cap the allocation passes 256 registers (1 wave per SIMD; the
+// hierarchical form ran 16 scenes in 1,223 ms there, 751 ms at 2).  With
+// registers to spare the guarded march queues its FP64 re-evaluations per
+// wave (march_ray.hpp), 24 % faster on the 32-scene batch.
 template <int PREC, int INTEG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(INTEG == INT_SIMPLE ? 2 : 1))) void batch_kernel(BatchGroup g,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(BatchGroup g,
                                                     const MarchParams* __restrict__ jobs) {
   const int lane = threadIdx.x & 63;
   const int tile_i = 8 * g.wave_rows, tile_j = 8 * g.wave_cols;
