@@ -40,13 +40,13 @@ namespace xrp {
 // scenes than inline re-evaluation at 4 waves, the same bits.
 // Hierarchical: 3 waves (168 VGPRs, the toggle queue's state stays in
 // registers), 7 % faster than 4 on the lattice.
-// Ray tile of one wave (i along the image's horizontal axis): 32 x 2 for
-// the simple integrator, whose rays then lie along one horizontal line per
-// sample index (lattice 39.3 -> 33.2 ms, gyroid lattice 2.57 -> 2.25 ms
-// against 8 x 8 tiles; CSG, launch-bound, 0.091 -> 0.094 ms); 8 x 8 for the
-// hierarchical integrator, where 32 x 2 was not measured.
-constexpr int march_tile_i(int integ) { return integ == INT_SIMPLE ? 32 : 8; }
-constexpr int march_tile_j(int integ) { return integ == INT_SIMPLE ? 2 : 8; }
+// Ray tile of one wave: 32 pixels along i (the image's horizontal axis) by
+// 2 along j, so a wave's samples of one index lie along a horizontal line.
+// Against 8 x 8 tiles: simple lattice 39.3 -> 33.2 ms, gyroid lattice 2.57
+// -> 2.25 ms, CSG (launch-bound) 0.091 -> 0.094 ms; hierarchical lattice
+// 104.8 -> 87.2 ms, hierarchical CSG 0.305 -> 0.229 ms.
+constexpr int march_tile_i(int) { return 32; }
+constexpr int march_tile_j(int) { return 2; }
 
 template <int PREC, int INTEG, int B>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
