@@ -44,54 +44,90 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+def _bind(L):
+    """Declares the argument and result types of every exported function."""
+    P = ctypes.c_void_p
+    D = ctypes.c_double
+    DP = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.c_int
+    sig = {
+        "orc_sphere": (P, [DP, D, D]),
+        "orc_cube": (P, [DP, D, D]),
+        "orc_box": (P, [DP, DP, D]),
+        "orc_parallelepiped": (P, [DP, DP, DP, DP, D]),
+        "orc_cylinder": (P, [DP, DP, D, D]),
+        "orc_gyroid": (P, [DP, D, D, D]),
+        "orc_collection": (P, [I]),
+        "orc_collection_add": (I, [P, P]),
+        "orc_unit_cell": (P, [P, DP]),
+        "orc_tessellated": (P, [P, DP]),
+        "orc_voxel_grid": (P, [DP, I, I, I]),
+        "orc_node_free": (None, [P]),
+        "orc_density": (D, [P, D, D, D]),
+        "orc_min_feature_size": (D, [P]),
+        "orc_def_gaussian": (P, [DP, DP, DP]),
+        "orc_def_affine": (P, [DP]),
+        "orc_def_linear": (P, [DP]),
+        "orc_def_rigid": (P, [DP]),
+        "orc_def_sigmoid": (P, [D, D, D, I]),
+        "orc_def_tanh": (P, [D, D, I, I]),
+        "orc_def_composed": (P, []),
+        "orc_def_composed_add": (I, [P, P]),
+        "orc_deform_free": (None, [P]),
+        "orc_deform_apply": (None, [P, DP]),
+        "orc_scene_density": (D, [P, D, D, D]),
+        "orc_integrate_simple": (D, [P, DP, DP, D, D, D]),
+        "orc_integrate_hierarchical": (D, [P, DP, DP, D, D, D]),
+        "orc_camera_from_angles": (None, [D, D, D, DP, DP]),
+        "orc_pixel_ray": (None, [DP, DP, I, D, I, I, DP, DP]),
+        "orc_render": (None, [P, DP, DP, I, I, D, D, D, I, I, I, I, DP]),
+        "orc_export_volume": (None, [P, I, I, DP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.c_void_p
-        D = ctypes.c_double
-        DP = ctypes.POINTER(ctypes.c_double)
-        I = ctypes.c_int
-        sig = {
-            "orc_sphere": (P, [DP, D, D]),
-            "orc_cube": (P, [DP, D, D]),
-            "orc_box": (P, [DP, DP, D]),
-            "orc_parallelepiped": (P, [DP, DP, DP, DP, D]),
-            "orc_cylinder": (P, [DP, DP, D, D]),
-            "orc_gyroid": (P, [DP, D, D, D]),
-            "orc_collection": (P, [I]),
-            "orc_collection_add": (I, [P, P]),
-            "orc_unit_cell": (P, [P, DP]),
-            "orc_tessellated": (P, [P, DP]),
-            "orc_voxel_grid": (P, [DP, I, I, I]),
-            "orc_node_free": (None, [P]),
-            "orc_density": (D, [P, D, D, D]),
-            "orc_min_feature_size": (D, [P]),
-            "orc_def_gaussian": (P, [DP, DP, DP]),
-            "orc_def_affine": (P, [DP]),
-            "orc_def_linear": (P, [DP]),
-            "orc_def_rigid": (P, [DP]),
-            "orc_def_sigmoid": (P, [D, D, D, I]),
-            "orc_def_tanh": (P, [D, D, I, I]),
-            "orc_def_composed": (P, []),
-            "orc_def_composed_add": (I, [P, P]),
-            "orc_deform_free": (None, [P]),
-            "orc_deform_apply": (None, [P, DP]),
-            "orc_scene_density": (D, [P, D, D, D]),
-            "orc_integrate_simple": (D, [P, DP, DP, D, D, D]),
-            "orc_integrate_hierarchical": (D, [P, DP, DP, D, D, D]),
-            "orc_camera_from_angles": (None, [D, D, D, DP, DP]),
-            "orc_pixel_ray": (None, [DP, DP, I, D, I, I, DP, DP]),
-            "orc_render": (None, [P, DP, DP, I, I, D, D, D, I, I, I, I, DP]),
-            "orc_export_volume": (None, [P, I, I, DP]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def build_mutant(mutant: int, out_dir: str):
+    """Compiles xrp_oracle.c with -DORC_MUTANT=mutant (a planted mistake,
+    listed at the top of that file) into out_dir and returns the loaded
+    library.  Used only by the tests that show the oracle's pins reject
+    each planted mistake."""
+    path = os.path.join(out_dir, f"liboracle_mutant{mutant}.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+                    "-std=c11", "-D_GNU_SOURCE", f"-DORC_MUTANT={int(mutant)}",
+                    "-shared", "-o", path, os.path.join(HERE, "xrp_oracle.c"),
+                    "-lm", "-lpthread"], check=True)
+    return _bind(ctypes.CDLL(path))
+
+
+class using_library:
+    """Routes every oracle call made inside the block to library L (a
+    mutant from build_mutant); objects built inside must be dropped inside."""
+
+    def __init__(self, L):
+        self.L = L
+
+    def __enter__(self):
+        global _lib
+        lib()
+        self.saved, _lib = _lib, self.L
+        return self.L
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 def _vec(vals: Sequence[float], n: int = 3):

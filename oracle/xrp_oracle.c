@@ -15,7 +15,7 @@
  * Mat4.Det/Inv by cofactor expansion, TransformCoordinate, Mat3.Inv,
  * Vec3.Normalize).  Their last-ulp rounding versus mathgl is parity
  * unpinned; the camera is pinned against the closed-form gluLookAt basis
- * in tests/test_oracle_camera.py.  Go's math.Sin/Cos/Exp are replaced by
+ * in tests/test_oracle_closed_forms.py.  Go's math.Sin/Cos/Exp are replaced by
  * the C library's (both within 1 ulp of the true value).
  */
 #include "xrp_oracle.h"
@@ -28,6 +28,20 @@
 
 /* main.go:46 */
 #define CUBE_HALF_DIAGONAL 1.74
+
+/* Planted mistakes for tests/test_oracle_hierarchical.py, which compiles
+ * this file with -DORC_MUTANT=n into a temporary library and checks that
+ * the integrator pins reject each one.  0 (the default, and the only value
+ * oracle/Makefile uses) is the faithful restatement.
+ *   1  hierarchical: the right endpoint's rho*ds term dropped (main.go:191)
+ *   2  hierarchical: first fine sample at left, before left += ds (main.go:182)
+ *   3  hierarchical: fine walk up to and including right (main.go:183)
+ *   4  hierarchical: rho*DS added on a transition as well (main.go:192-193)
+ *   5  simple: samples while s <= smax (main.go:147)
+ *   6  simple: flat field left out (main.go:146) */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 
 enum {
   N_SPHERE = 1, N_CUBE, N_BOX, N_PPED, N_CYL, N_GYROID,
@@ -530,8 +544,16 @@ double orc_integrate_simple(const orc_scene *sc, const double origin[3],
                             double smax) {
   double d[3];
   normalize(direction, d);
+#if ORC_MUTANT == 6
+  double T = 0.0;
+#else
   double T = sc->flat_field;
+#endif
+#if ORC_MUTANT == 5
+  for (double s = smin; s <= smax; s += ds) {
+#else
   for (double s = smin; s < smax; s += ds) {
+#endif
     double x = origin[0] + d[0] * s;
     double y = origin[1] + d[1] * s;
     double z = origin[2] + d[2] * s;
@@ -558,15 +580,26 @@ double orc_integrate_hierarchical(const orc_scene *sc, const double origin[3],
     double z = origin[2] + d[2] * right;
     double rho = orc_scene_density(sc, x, y, z);
     if ((rho == 0) != (prev_rho == 0)) {
+#if ORC_MUTANT != 2
       left += ds;
+#endif
+#if ORC_MUTANT == 3
+      while (left <= right) {
+#else
       while (left < right) {
+#endif
         double xl = origin[0] + d[0] * left;
         double yl = origin[1] + d[1] * left;
         double zl = origin[2] + d[2] * left;
         T += orc_scene_density(sc, xl, yl, zl) * ds;
         left += ds;
       }
+#if ORC_MUTANT != 1
       T += rho * ds;
+#endif
+#if ORC_MUTANT == 4
+      T += rho * DS;
+#endif
     } else {
       T += rho * DS;
     }
