@@ -1,9 +1,14 @@
-"""Benchmark: 512^2 projections/s on one MI355X.
+"""Benchmark: 512^2 projections/s and ray-samples/s on one MI355X, plain vs
+surrogate.
 
-Default workload (one step = one render of all views, inputs resident):
-  voxel grid 256^3 float32 (64 seeded Gaussian blobs, world [-0.5,0.5]^3),
-  512x512 pixels, 64 views at random azimuth / polar 90, simple integrator,
-  ds = 2/256/5.  Other workloads: --workload csg|lattice|gyroid.
+Default workload (one step = one render of all views, inputs resident in
+HBM): the tessellated gyroid, 6^3 cells of one period each, thickness 0.1,
+rho 1, under z += 0.1 tanh(8x); 1024x1024 pixels, 256 views at azimuth
+k*360/256, polar 90, simple integrator, ds 0.001.  The plain form (guarded
+FP32 march, exact against the FP64 oracle) gives `value`; the surrogate form
+(hash-grid MLP trained on 2^24 exact samples, 16x16 tiles checked against
+the exact march and re-rendered above 1e-3) is timed beside it.  Other
+workloads: --workload voxel|csg|lattice|gyroid|batch|surrogate.
 
 Prints one JSON line with the throughput, the dominant kernel's roofline
 position (HIP-event kernel time, algorithmic bytes or flops per sample)
@@ -81,6 +86,46 @@ def inbox_samples(w: W.Workload, lo: np.ndarray, hi: np.ndarray) -> int:
         a = np.searchsorted(s, tn[ok], side="left")
         b = np.searchsorted(s, tf[ok], side="right")
         total += int(np.sum(b - a))
+    return total
+
+
+def inbox_samples_device(w: W.Workload, lo: np.ndarray, hi: np.ndarray) -> int:
+    """inbox_samples with the per-view arithmetic in FP64 on the device
+    (torch), for workloads with many large views."""
+    import torch
+    dev = "cuda:0"
+    smin, smax = w.R - 1.74, w.R + 1.74
+    s, x = [], smin
+    while x < smax:
+        s.append(x)
+        x += w.ds
+    s = torch.tensor(s, dtype=torch.float64, device=dev)
+    res = w.resolution
+    half = res / 2.0
+    f = 1.0 / math.tan(w.fov / 2 * math.pi / 180)
+    ar = torch.arange(res, dtype=torch.float64, device=dev)
+    ii, jj = torch.meshgrid(ar, ar, indexing="ij")
+    v = torch.stack([ii / half - 1.0, jj / half - 1.0, torch.full_like(ii, -f),
+                     torch.ones_like(ii)], -1).reshape(-1, 4)
+    lo_t = torch.tensor(lo, dtype=torch.float64, device=dev)
+    hi_t = torch.tensor(hi, dtype=torch.float64, device=dev)
+    total = 0
+    for az, po in w.angles:
+        eye, cam = X.camera_from_angles(az, po, w.R)
+        cam_t = torch.tensor(cam, dtype=torch.float64, device=dev)
+        eye_t = torch.tensor(eye, dtype=torch.float64, device=dev)
+        t = v @ cam_t.T
+        p = t[:, :3] / t[:, 3:4]
+        d = p - eye_t
+        d = d / torch.linalg.norm(d, dim=1, keepdim=True)
+        t1 = (lo_t - eye_t) / d
+        t2 = (hi_t - eye_t) / d
+        tn = torch.nan_to_num(torch.minimum(t1, t2), nan=-math.inf).amax(dim=1)
+        tf = torch.nan_to_num(torch.maximum(t1, t2), nan=math.inf).amin(dim=1)
+        ok = tn <= tf
+        a = torch.searchsorted(s, tn[ok].contiguous(), right=False)
+        b = torch.searchsorted(s, tf[ok].contiguous(), right=True)
+        total += int((b - a).sum().item())
     return total
 
 
@@ -371,12 +416,149 @@ def bench_surrogate(args) -> int:
     return 0
 
 
+def measured_counter(name: str, kernel: str, workload: str):
+    """A per-launch figure recorded by a rocprofv3 summary under profiles/
+    as a line `<name>: <kernel> | <workload> | <value>`, or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for fn in sorted(os.listdir(d)):
+        with open(os.path.join(d, fn), errors="replace") as fh:
+            for line in fh:
+                if not line.startswith(name + ":"):
+                    continue
+                parts = [x.strip() for x in line.split(":", 1)[1].split("|")]
+                if len(parts) == 3 and parts[0] == kernel and parts[1] == workload:
+                    return float(parts[2])
+    return None
+
+
+C5_WORKLOAD = "gyroid6_tanh_256x1024"
+
+
+def bench_c5(args) -> int:
+    """The metric's configuration: tessellated gyroid 6^3 + z += 0.1 tanh(8x),
+    256 views x 1024^2, ds 0.001.  Plain form = the guarded FP32 march
+    (`value`); surrogate form = hash-grid MLP + exact tile fallback at 1e-3."""
+    import torch
+    n_views = args.views or 256
+    w = W.gyroid_lattice(resolution=1024, n_views=n_views, ds=0.001, shear=True)
+    res = w.resolution
+    ctx = X.Context(0)
+    sc = X.Scene.from_dict(w.obj, w.deformation)
+    lo, hi, empty = sc.clip_box()
+    exact = torch.empty((n_views, res, res), dtype=torch.float32, device="cuda:0")
+    kw = dict(integrator="simple", precision=args.precision, R=w.R, fov=w.fov)
+    for _ in range(args.warmup):
+        st = X.render_device(ctx, sc, w.angles, res, w.ds, exact.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    print("c5: plain warm-up done", file=sys.stderr, flush=True)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = X.render_device(ctx, sc, w.angles, res, w.ds, exact.data_ptr(), **kw)
+        kernel_ms.append(st["kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    print(f"c5: plain {args.steps} steps, {ms_per_step:.1f} ms each", file=sys.stderr,
+          flush=True)
+    proj = w.pixels / (512.0 * 512.0)
+    value = proj / (ms_per_step / 1000.0)
+    n_inbox = inbox_samples_device(w, lo, hi)
+    k_ms = float(np.mean(kernel_ms))
+    kernel = f"march_kernel<PREC_GUARDED,INT_SIMPLE,{4 if sc.layout()[0] >= 24 else 2}>"
+    achieved = FLOPS_PER_SAMPLE["gyroid"] * n_inbox / (k_ms / 1000.0) / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": measured_traffic(kernel, C5_WORKLOAD), "kernel": kernel,
+            "kernel_ms": k_ms, "per_sample": f"{FLOPS_PER_SAMPLE['gyroid']:.0f} flop + 8 "
+                                             "transcendental (cost table)",
+            "samples_per_launch": n_inbox, "evaluated_samples": st["samples"] -
+            st["skipped_samples"], "skipped_samples": st["skipped_samples"]}
+    insts = measured_counter("valu_insts_per_launch", kernel, C5_WORKLOAD)
+    if insts:
+        # executed VALU issue: wave instructions x 4 cycles over the SIMDs'
+        # cycles (1024 SIMDs at 2.4 GHz) in the launch
+        roof["valu_issue_frac"] = insts * 4.0 / (k_ms / 1000.0 * 1024 * 2.4e9)
+    line = {
+        "metric": METRIC, "value": value, "unit": "512² projections/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"gyroid 6^3 + z += 0.1 tanh(8x): {n_views} views x {res}^2, "
+                               f"ds={w.ds:g}", "form": "plain (guarded FP32 march)",
+                   "precision": {1: "guarded", 2: "fp64"}.get(st["precision_used"], "auto"),
+                   "integrator": "simple"},
+        "ray_samples_per_s": n_inbox / (ms_per_step / 1000.0),
+        "inbox_samples": n_inbox,
+        "reevaluated": st["reevaluated"],
+        "roofline": roof,
+    }
+    if not args.no_surrogate:
+        line["surrogate"] = c5_surrogate_leg(args, ctx, sc, w, exact, kw, ms_per_step)
+    if not args.no_cpu:
+        line["cpu_baseline"], images = cpu_baseline(w)
+        err, n = 0.0, 0
+        for v, r0, ref in images:
+            got = exact[v, r0:r0 + ref.shape[0]].double().cpu().numpy()
+            err = max(err, float(np.max(np.abs(got - ref))))
+            n += ref.size
+        line["error_vs_oracle"] = {"max_abs": err, "pixels": n, "tolerance": 1e-5}
+    print(json.dumps(line))
+    sc.close()
+    ctx.close()
+    return 0
+
+
+def c5_surrogate_leg(args, ctx, sc, w, exact, kw, plain_ms: float) -> dict:
+    """Trains the surrogate (2^24 exact samples) and renders every view with
+    the 1e-3 tile check and exact fallback; error against the plain render
+    (itself checked against the oracle)."""
+    import torch
+    res, n_views = w.resolution, len(w.angles)
+    sur = X.Surrogate(ctx, sc, log2_table=19, n_min=16, n_max=512)
+    tr = sur.train(samples=1 << 24, batch=65536, epochs=4)
+    print(f"c5: surrogate trained, loss {tr['first_loss']:.3g} -> {tr['final_loss']:.3g}",
+          file=sys.stderr, flush=True)
+    img = torch.empty_like(exact)
+    ds_sur, tol = 2 * w.ds, 1e-3
+    sur.render_device(w.angles, res, w.ds, ds_sur, tol, img.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    steps = max(1, args.surrogate_steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = sur.render_device(w.angles, res, w.ds, ds_sur, tol, img.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    print(f"c5: surrogate {steps} steps, {ms:.0f} ms each", file=sys.stderr, flush=True)
+    diff = (img - exact).abs()
+    out = {
+        "value": w.pixels / (512.0 * 512.0) / (ms / 1000.0), "unit": "512² projections/s",
+        "ms_per_step": ms, "steps": steps, "speedup_vs_plain": plain_ms / ms,
+        "error_vs_plain": {"max_abs": float(diff.max()), "mean_abs": float(diff.mean()),
+                           "pixels_above_tolerance": int((diff > tol).sum()),
+                           "tolerance": tol},
+        "tiles": st["tiles"], "tiles_rerendered": st["tiles_failed"],
+        "check_pixels": st["check_pixels"], "network_samples": st["network_samples"],
+        "surrogate_ms": st["surrogate_ms"], "check_ms": st["check_ms"],
+        "fallback_ms": st["fallback_ms"], "train": tr,
+        "config": f"step {ds_sur:g} (midpoint), tile tolerance {tol:g}",
+    }
+    sur.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="voxel")
+    ap.add_argument("--workload", default="c5")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="c5: time the plain form only")
+    ap.add_argument("--surrogate-steps", type=int, default=1,
+                    help="c5: timed renders of the surrogate form")
     ap.add_argument("--precision", default="auto")
     ap.add_argument("--integrator", default="", help="override the workload's integrator")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -403,6 +585,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "surrogate":
         return bench_surrogate(args)
+    if args.workload == "c5":
+        return bench_c5(args)
 
     import torch
     w = make_workload(args.workload)

@@ -23,7 +23,9 @@
 extern "C" {
 #endif
 
-#define XRP_ABI_VERSION 1
+/* 2: every output stats struct opens with struct_size (set by the caller);
+ *    XRP_ERR_INTERNAL added. */
+#define XRP_ABI_VERSION 2
 
 /* Status codes.  1 and 2 keep the meaning of cuda_backend.cu:95-100. */
 typedef enum {
@@ -36,7 +38,8 @@ typedef enum {
   XRP_ERR_DEFORMATION = 11,/* deformation description rejected */
   XRP_ERR_ARG = 12,        /* invalid parameter value (ds <= 0, unknown enum, ...) */
   XRP_ERR_UNSUPPORTED = 13,/* valid for the reference, not supported by this library */
-  XRP_ERR_IO = 14          /* file could not be read or written */
+  XRP_ERR_IO = 14,         /* file could not be read or written */
+  XRP_ERR_INTERNAL = 15    /* unexpected library failure (message says which) */
 } xrp_status;
 
 /* Integrators: main.go:144-154 and main.go:159-199. */
@@ -103,7 +106,15 @@ typedef enum { XRP_OUTPUT_F32 = 0, XRP_OUTPUT_GRAY8_ALPHA8 = 1 } xrp_output_form
  * R 4, fov 40, flat_field 0, density_multiplier 1, hierarchical. */
 void xrp_default_params(xrp_render_params *p);
 
+/* Output structs (xrp_render_stats, xrp_batch_stats, xrp_train_stats,
+ * xrp_surrogate_stats) open with struct_size: the caller sets it to
+ * sizeof(the struct it was compiled with) and the library writes no byte
+ * beyond it, so fields appended later never overrun an older caller.  A
+ * struct_size below 8 is rejected with XRP_ERR_ARG; a NULL pointer asks for
+ * no statistics. */
 typedef struct {
+  uint32_t struct_size;
+  int32_t reserved0;
   uint64_t rays;
   uint64_t samples;        /* samples marched after bounding-box clipping (evaluated or
                               settled by a safe radius, see skipped_samples) */
@@ -230,6 +241,8 @@ typedef struct {
 } xrp_tile_shape;
 
 typedef struct {
+  uint32_t struct_size; /* sizeof(xrp_batch_stats), set by the caller */
+  int32_t reserved0;
   double kernel_ms;     /* graph replay, HIP events on the context stream */
   uint64_t rays;
   uint64_t samples;
@@ -294,6 +307,8 @@ typedef struct {
 } xrp_train_config;
 
 typedef struct {
+  uint32_t struct_size; /* sizeof(xrp_train_stats), set by the caller */
+  int32_t reserved0;
   double first_loss;  /* mean squared error of the first step */
   double final_loss;  /* mean over the last epoch */
   double label_ms;    /* exact sample generation */
@@ -303,6 +318,8 @@ typedef struct {
 } xrp_train_stats;
 
 typedef struct {
+  uint32_t struct_size;  /* sizeof(xrp_surrogate_stats), set by the caller */
+  int32_t reserved0;
   double surrogate_ms;   /* surrogate march */
   double check_ms;       /* exact march of the check pixels */
   double fallback_ms;    /* exact re-render of failing tiles */
@@ -311,6 +328,8 @@ typedef struct {
   uint64_t tiles_failed;
   uint64_t check_pixels;
   uint64_t network_samples; /* network evaluations issued by the surrogate march */
+  uint64_t exact_samples;   /* samples the exact check and fallback marches evaluated
+                               (in the clip box, less those settled by a safe radius) */
 } xrp_surrogate_stats;
 
 void xrp_surrogate_default_config(xrp_surrogate_config *c);
@@ -321,8 +340,10 @@ int xrp_surrogate_train(xrp_surrogate *s, const xrp_train_config *config,
                         xrp_train_stats *stats);
 /* Renders with the network at step ds_surrogate into device memory (float32
  * transmittance [view][i][j]); with tile_tolerance > 0 every 16x16 tile is
- * checked at 4 pixels against the exact march (params->ds, integrator,
- * precision) and re-rendered exactly when any differs by more. */
+ * checked at 16 pixels (a stride-4 grid) against the exact march
+ * (params->ds, integrator, precision) and re-rendered exactly when any
+ * differs by more than tile_tolerance / 2.  The check is a sample: pixels
+ * between check pixels of a tile that passed are not compared. */
 int xrp_surrogate_render_device(xrp_surrogate *s, const xrp_render_params *params,
                                 const xrp_view_angle *views, int n_views,
                                 double ds_surrogate, double tile_tolerance,

@@ -2,6 +2,11 @@ import os
 import sys
 
 import pytest
+# torch ships HIP and HSA runtimes with the same sonames as the system ROCm
+# that libxrp.so links (libamdhip64.so.7, libhsa-runtime64.so.1); the first
+# loaded serves the whole process, and torch initialises only on its own.
+# Loading torch before libxrp gives every test the runtime bench.py uses.
+import torch  # noqa: F401,E402
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)

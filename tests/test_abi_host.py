@@ -41,7 +41,7 @@ def test_exports_every_declared_function():
 
 
 def test_abi_version():
-    assert X.lib().xrp_abi_version() == 1
+    assert X.lib().xrp_abi_version() == 2
 
 
 @pytest.mark.parametrize("az,po,R", [(0, 90, 4), (45, 60, 5), (123.4, 77.7, 4), (315, 90, 4),

@@ -112,15 +112,77 @@ def test_training_and_render_with_exact_fallback(ctx):
     tol = 1e-3
     img, rs = sur.render(w.angles, 64, w.ds, 2 * w.ds, tol, **kw)
     print("render", rs, flush=True)
-    assert rs["tiles"] == 2 * 4 * 4 and rs["check_pixels"] == 4 * rs["tiles"]
-    # every check pixel of a tile that was not re-rendered is within tol
+    assert rs["tiles"] == 2 * 4 * 4 and rs["check_pixels"] == 16 * rs["tiles"]
+    # every check pixel of a tile that was not re-rendered is within tol / 2
     for v in range(2):
         for ti in range(4):
             for tj in range(4):
-                pts = [(ti * 16 + a, tj * 16 + b) for a, b in ((3, 3), (3, 11), (11, 3), (11, 11))]
+                pts = [(ti * 16 + a, tj * 16 + b) for a in (1, 5, 9, 13) for b in (1, 5, 9, 13)]
                 d = max(abs(img[v, i, j] - exact[v, i, j]) for i, j in pts)
                 tile = img[v, ti * 16:(ti + 1) * 16, tj * 16:(tj + 1) * 16]
                 ref = exact[v, ti * 16:(ti + 1) * 16, tj * 16:(tj + 1) * 16]
-                assert d <= tol + 1e-6 or np.array_equal(tile, ref)
+                assert d <= 0.5 * tol + 1e-6 or np.array_equal(tile, ref)
     assert rs["max_sampled_error"] >= 0.0
     sur.close()
+
+
+@pytest.fixture(scope="module")
+def c5_reduced(ctx):
+    """The surrogate workload at reduced size: the sheared gyroid lattice
+    at the metric's step (ds 0.001) and network (16 levels, 2^19 entries,
+    resolutions 16..512), 4 views at 128^2; the exact march renders the
+    reference image (itself held to the oracle by the parity tests)."""
+    w = W.gyroid_lattice(resolution=128, n_views=4, ds=0.001, shear=True)
+    w.angles = [(10.0, 90.0), (75.0, 90.0), (160.0, 90.0), (300.0, 90.0)]
+    sc = xrp_scene(w.obj, w.deformation)
+    kw = dict(integrator="simple")
+    exact, est = X.render(ctx, sc, w.angles, 128, w.ds, **kw)
+    sur = X.Surrogate(ctx, sc, log2_table=19, n_min=16, n_max=512)
+    untrained = X.Surrogate(ctx, sc, log2_table=19, n_min=16, n_max=512, seed=2)
+    tr = sur.train(samples=1 << 22, batch=1 << 16, epochs=4)
+    print("train", tr, flush=True)
+    yield w, sc, exact, est, sur, untrained
+    untrained.close()
+    sur.close()
+    sc.close()
+
+
+def test_surrogate_with_fallback_within_tolerance_everywhere(c5_reduced):
+    """After the tile check and exact fallback, every pixel of the image is
+    within 1e-3 of the exact render (not only the check pixels)."""
+    w, sc, exact, _, sur, _ = c5_reduced
+    tol = 1e-3
+    img, rs = sur.render(w.angles, 128, w.ds, 2 * w.ds, tol, integrator="simple")
+    err = np.abs(img.astype(np.float64) - exact)
+    print(f"full image max {err.max():.3e}; tiles {rs['tiles']} re-rendered "
+          f"{rs['tiles_failed']}; {rs}", flush=True)
+    assert float(err.max()) <= tol
+    pure, _ = sur.render(w.angles, 128, w.ds, 2 * w.ds, 0.0, integrator="simple")
+    perr = np.abs(pure.astype(np.float64) - exact)
+    print(f"pure surrogate max {perr.max():.3e}, pixels above tol {(perr > tol).mean():.3%}",
+          flush=True)
+
+
+def test_surrogate_ab_against_exact_only(c5_reduced):
+    """A/B of the surrogate form (network march + exact check + fallback)
+    against the exact-only march: time and samples evaluated by each, and
+    what training buys: the trained network sends fewer tiles to the exact
+    fallback than an untrained one, and both results meet the tolerance."""
+    w, sc, exact, est, sur, untrained = c5_reduced
+    tol = 1e-3
+    img_t, st_t = sur.render(w.angles, 128, w.ds, 2 * w.ds, tol, integrator="simple")
+    img_u, st_u = untrained.render(w.angles, 128, w.ds, 2 * w.ds, tol, integrator="simple")
+    exact_eval = est["samples"] - est["skipped_samples"]
+    for name, st in (("trained", st_t), ("untrained", st_u)):
+        ms = st["surrogate_ms"] + st["check_ms"] + st["fallback_ms"]
+        print(f"{name}: {ms:.2f} ms (network {st['surrogate_ms']:.2f}, check "
+              f"{st['check_ms']:.2f}, fallback {st['fallback_ms']:.2f}); network samples "
+              f"{st['network_samples']}, exact samples {st['exact_samples']}; tiles "
+              f"re-rendered {st['tiles_failed']}/{st['tiles']}", flush=True)
+    print(f"exact only: {est['kernel_ms']:.2f} ms; samples evaluated {exact_eval} "
+          f"(+{est['skipped_samples']} settled)", flush=True)
+    assert float(np.abs(img_t.astype(np.float64) - exact).max()) <= tol
+    assert float(np.abs(img_u.astype(np.float64) - exact).max()) <= tol
+    assert st_t["tiles_failed"] < st_u["tiles_failed"]
+    assert st_t["exact_samples"] < st_u["exact_samples"]
+    assert st_t["network_samples"] == st_u["network_samples"] > 0

@@ -21,7 +21,7 @@ XRP_OK = 0
 STATUS_NAMES = {
     0: "XRP_OK", 1: "XRP_ERR_NULL", 2: "XRP_ERR_DIMS", 3: "XRP_ERR_DEVICE",
     4: "XRP_ERR_ALLOC", 10: "XRP_ERR_SCENE", 11: "XRP_ERR_DEFORMATION",
-    12: "XRP_ERR_ARG", 13: "XRP_ERR_UNSUPPORTED", 14: "XRP_ERR_IO",
+    12: "XRP_ERR_ARG", 13: "XRP_ERR_UNSUPPORTED", 14: "XRP_ERR_IO", 15: "XRP_ERR_INTERNAL",
 }
 INTEGRATORS = {"simple": 0, "hierarchical": 1}
 PRECISIONS = {"auto": 0, "guarded": 1, "fp64": 2}
@@ -56,8 +56,22 @@ class RenderParams(ctypes.Structure):
     ]
 
 
-class RenderStats(ctypes.Structure):
+class _OutStruct(ctypes.Structure):
+    """Output structs open with struct_size, set here to the size this
+    binding declares (the library writes no further)."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        self.struct_size = ctypes.sizeof(self)
+
+    def as_dict(self) -> Dict:
+        return {k: getattr(self, k) for k, _ in self._fields_
+                if k not in ("reserved", "reserved0", "struct_size")}
+
+
+class RenderStats(_OutStruct):
     _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("reserved0", ctypes.c_int32),
         ("rays", ctypes.c_uint64), ("samples", ctypes.c_uint64),
         ("reevaluated", ctypes.c_uint64), ("audit_mismatch", ctypes.c_uint64),
         ("fine_samples", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
@@ -66,9 +80,6 @@ class RenderStats(ctypes.Structure):
         ("staged_voxels", ctypes.c_uint64), ("fallback_chunks", ctypes.c_uint64),
         ("skipped_samples", ctypes.c_uint64),
     ]
-
-    def as_dict(self) -> Dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class ViewAngle(ctypes.Structure):
@@ -97,14 +108,12 @@ class TileShape(ctypes.Structure):
         return (self.wave_rows, self.wave_cols, self.groups_per_cu)
 
 
-class BatchStats(ctypes.Structure):
-    _fields_ = [("kernel_ms", ctypes.c_double), ("rays", ctypes.c_uint64),
+class BatchStats(_OutStruct):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("reserved0", ctypes.c_int32),
+                ("kernel_ms", ctypes.c_double), ("rays", ctypes.c_uint64),
                 ("samples", ctypes.c_uint64), ("reevaluated", ctypes.c_uint64),
                 ("tiles", ctypes.c_uint64), ("launches", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
-
-    def as_dict(self) -> Dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class SurrogateConfig(ctypes.Structure):
@@ -120,23 +129,20 @@ class TrainConfig(ctypes.Structure):
                 ("lr_mlp", ctypes.c_double)]
 
 
-class TrainStats(ctypes.Structure):
-    _fields_ = [("first_loss", ctypes.c_double), ("final_loss", ctypes.c_double),
+class TrainStats(_OutStruct):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("reserved0", ctypes.c_int32),
+                ("first_loss", ctypes.c_double), ("final_loss", ctypes.c_double),
                 ("label_ms", ctypes.c_double), ("train_ms", ctypes.c_double),
                 ("steps", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
-    def as_dict(self) -> Dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
-
-class SurrogateStats(ctypes.Structure):
-    _fields_ = [("surrogate_ms", ctypes.c_double), ("check_ms", ctypes.c_double),
+class SurrogateStats(_OutStruct):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("reserved0", ctypes.c_int32),
+                ("surrogate_ms", ctypes.c_double), ("check_ms", ctypes.c_double),
                 ("fallback_ms", ctypes.c_double), ("max_sampled_error", ctypes.c_double),
                 ("tiles", ctypes.c_uint64), ("tiles_failed", ctypes.c_uint64),
-                ("check_pixels", ctypes.c_uint64), ("network_samples", ctypes.c_uint64)]
-
-    def as_dict(self) -> Dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+                ("check_pixels", ctypes.c_uint64), ("network_samples", ctypes.c_uint64),
+                ("exact_samples", ctypes.c_uint64)]
 
 
 def lib() -> ctypes.CDLL:

@@ -8,6 +8,7 @@
 #include <functional>
 #include <map>
 #include <mutex>
+#include <new>
 #include <set>
 #include <string>
 #include <vector>
@@ -24,6 +25,41 @@ thread_local std::string g_error;
 int fail(int code, const std::string& msg) {
   g_error = msg;
   return code;
+}
+
+// Every int-returning entry point runs its body through this: no C++
+// exception crosses the C ABI (a host allocation that fails, for instance,
+// becomes XRP_ERR_ALLOC).
+template <class F>
+int guarded(F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(XRP_ERR_ALLOC, "host memory allocation failed");
+  } catch (const std::exception& e) {
+    return fail(XRP_ERR_INTERNAL, std::string("internal error: ") + e.what());
+  } catch (...) {
+    return fail(XRP_ERR_INTERNAL, "internal error");
+  }
+}
+
+// Output structs open with {uint32_t struct_size; int32_t reserved}: the
+// caller states the size it allocated and the library writes no further,
+// so fields appended later never overrun an older caller's struct.
+constexpr size_t kOutHeader = 8;
+
+template <class T>
+int check_out(const T* st, const char* name) {
+  if (st && st->struct_size < kOutHeader)
+    return fail(XRP_ERR_ARG, std::string(name) + ".struct_size is not set");
+  return XRP_OK;
+}
+
+template <class T>
+void write_out(T* dst, const T& src) {
+  if (!dst) return;
+  const size_t n = std::min<size_t>(dst->struct_size, sizeof(T));
+  std::memcpy((char*)dst + kOutHeader, (const char*)&src + kOutHeader, n - kOutHeader);
 }
 
 // Pinned host staging buffer (hipHostMalloc), grown on demand.
@@ -47,9 +83,15 @@ struct HostBuf {
   }
 };
 
+// Device buffer owned by exactly one holder: released explicitly (with its
+// device current) or, at the latest, by the destructor.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap && p) return hipSuccess;
     if (p) (void)hipFree(p);
@@ -68,6 +110,9 @@ struct DevBuf {
 };
 
 std::mutex g_registry_mu;  // scene uploads <-> contexts
+
+// Largest voxel grid accepted from caller memory (32 GiB of FP32).
+constexpr double kMaxVoxels = 8589934592.0;
 }  // namespace
 
 struct xrp_scene;
@@ -87,6 +132,9 @@ struct xrp_context {
   double table_smin = NAN, table_smax = NAN, table_ds = NAN;
   std::vector<double> table_host;
   std::set<xrp_scene*> scenes;  // scenes holding device copies here
+  // xrp_render_volume_projections: the last volume as a scene, reused while
+  // the caller passes the same grid
+  xrp_scene* vol_scene = nullptr;
 };
 
 struct xrp_scene {
@@ -139,31 +187,35 @@ void xrp_default_params(xrp_render_params* p) {
 }
 
 int xrp_context_create(int device, xrp_context** out) {
-  if (!out) return fail(XRP_ERR_NULL, "out is NULL");
-  *out = nullptr;
-  int n = 0;
-  HIPCHK(hipGetDeviceCount(&n));
-  if (device < 0 || device >= n)
-    return fail(XRP_ERR_ARG, "device " + std::to_string(device) +
-                                 " out of range (" + std::to_string(n) +
-                                 " devices)");
-  HIPCHK(hipSetDevice(device));
-  auto* c = new xrp_context();
-  c->device = device;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-  if (e != hipSuccess) {
-    delete c;
-    return fail(XRP_ERR_DEVICE, std::string("context set-up failed: ") +
-                                    hipGetErrorString(e));
-  }
-  *out = c;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!out) return fail(XRP_ERR_NULL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+      return fail(XRP_ERR_ARG, "device " + std::to_string(device) +
+                                   " out of range (" + std::to_string(n) +
+                                   " devices)");
+    HIPCHK(hipSetDevice(device));
+    auto* c = new xrp_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(XRP_ERR_DEVICE, std::string("context set-up failed: ") +
+                                      hipGetErrorString(e));
+    }
+    *out = c;
+    return XRP_OK;
+  });
 }
 
 void xrp_context_destroy(xrp_context* ctx) {
   if (!ctx) return;
+  if (ctx->vol_scene) xrp_scene_destroy(ctx->vol_scene);
+  ctx->vol_scene = nullptr;
   {
     std::lock_guard<std::mutex> g(g_registry_mu);
     for (xrp_scene* sc : ctx->scenes) free_uploads(sc, ctx);
@@ -182,10 +234,12 @@ void xrp_context_destroy(xrp_context* ctx) {
 }
 
 int xrp_context_synchronize(xrp_context* ctx) {
-  if (!ctx) return fail(XRP_ERR_NULL, "ctx is NULL");
-  HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!ctx) return fail(XRP_ERR_NULL, "ctx is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return XRP_OK;
+  });
 }
 
 static int parse_text(const char* text, xrp::json::Value& out, bool& present) {
@@ -201,61 +255,67 @@ static int parse_text(const char* text, xrp::json::Value& out, bool& present) {
 
 int xrp_scene_create(const char* object_text, const char* deformation_text,
                      int number_mode, const char* base_dir, xrp_scene** out) {
-  if (!out) return fail(XRP_ERR_NULL, "out is NULL");
-  *out = nullptr;
-  if (!object_text) return fail(XRP_ERR_NULL, "object_text is NULL");
-  if (number_mode != XRP_NUMBERS_JSON && number_mode != XRP_NUMBERS_YAML)
-    return fail(XRP_ERR_ARG, "unknown number_mode");
-  xrp::json::Value obj, def;
-  bool has_obj, has_def;
-  int rc = parse_text(object_text, obj, has_obj);
-  if (rc) return rc;
-  if (!has_obj) return fail(XRP_ERR_SCENE, "empty object description");
-  rc = parse_text(deformation_text, def, has_def);
-  if (rc) return rc;
-  auto* sc = new xrp_scene();
-  try {
-    sc->s = xrp::load_scene(obj, nullptr, (xrp::NumberMode)number_mode,
-                            base_dir ? base_dir : "");
-  } catch (const std::exception& e) {
-    delete sc;
-    return fail(XRP_ERR_SCENE, e.what());
-  }
-  if (has_def) {
+  return guarded([&]() -> int {
+    if (!out) return fail(XRP_ERR_NULL, "out is NULL");
+    *out = nullptr;
+    if (!object_text) return fail(XRP_ERR_NULL, "object_text is NULL");
+    if (number_mode != XRP_NUMBERS_JSON && number_mode != XRP_NUMBERS_YAML)
+      return fail(XRP_ERR_ARG, "unknown number_mode");
+    xrp::json::Value obj, def;
+    bool has_obj, has_def;
+    int rc = parse_text(object_text, obj, has_obj);
+    if (rc) return rc;
+    if (!has_obj) return fail(XRP_ERR_SCENE, "empty object description");
+    rc = parse_text(deformation_text, def, has_def);
+    if (rc) return rc;
+    auto* sc = new xrp_scene();
     try {
-      xrp::append_deformation(def, (xrp::NumberMode)number_mode, sc->s->defs);
-      xrp::finalize_scene(*sc->s);
+      sc->s = xrp::load_scene(obj, nullptr, (xrp::NumberMode)number_mode,
+                              base_dir ? base_dir : "");
     } catch (const std::exception& e) {
       delete sc;
-      return fail(XRP_ERR_DEFORMATION, e.what());
+      return fail(XRP_ERR_SCENE, e.what());
     }
-  }
-  *out = sc;
-  return XRP_OK;
+    if (has_def) {
+      try {
+        xrp::append_deformation(def, (xrp::NumberMode)number_mode, sc->s->defs);
+        xrp::finalize_scene(*sc->s);
+      } catch (const std::exception& e) {
+        delete sc;
+        return fail(XRP_ERR_DEFORMATION, e.what());
+      }
+    }
+    *out = sc;
+    return XRP_OK;
+  });
 }
 
 int xrp_scene_create_voxels(const float* rho, int nx, int ny, int nz,
                             const char* deformation_text, int number_mode,
                             xrp_scene** out) {
-  if (!out) return fail(XRP_ERR_NULL, "out is NULL");
-  *out = nullptr;
-  if (!rho) return fail(XRP_ERR_NULL, "rho is NULL");
-  if (nx <= 0 || ny <= 0 || nz <= 0)
-    return fail(XRP_ERR_DIMS, "voxel grid dimensions must be positive");
-  xrp::json::Value def;
-  bool has_def;
-  int rc = parse_text(deformation_text, def, has_def);
-  if (rc) return rc;
-  auto* sc = new xrp_scene();
-  try {
-    sc->s = xrp::voxel_scene(rho, nx, ny, nz, has_def ? &def : nullptr,
-                             (xrp::NumberMode)number_mode);
-  } catch (const std::exception& e) {
-    delete sc;
-    return fail(XRP_ERR_DEFORMATION, e.what());
-  }
-  *out = sc;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!out) return fail(XRP_ERR_NULL, "out is NULL");
+    *out = nullptr;
+    if (!rho) return fail(XRP_ERR_NULL, "rho is NULL");
+    if (nx <= 0 || ny <= 0 || nz <= 0)
+      return fail(XRP_ERR_DIMS, "voxel grid dimensions must be positive");
+    if ((double)nx * ny * nz > kMaxVoxels)
+      return fail(XRP_ERR_DIMS, "voxel grid larger than 2^33 voxels");
+    xrp::json::Value def;
+    bool has_def;
+    int rc = parse_text(deformation_text, def, has_def);
+    if (rc) return rc;
+    auto* sc = new xrp_scene();
+    try {
+      sc->s = xrp::voxel_scene(rho, nx, ny, nz, has_def ? &def : nullptr,
+                               (xrp::NumberMode)number_mode);
+    } catch (const std::exception& e) {
+      delete sc;
+      return fail(XRP_ERR_DEFORMATION, e.what());
+    }
+    *out = sc;
+    return XRP_OK;
+  });
 }
 
 void xrp_scene_destroy(xrp_scene* sc) {
@@ -273,54 +333,66 @@ void xrp_scene_destroy(xrp_scene* sc) {
 }
 
 int xrp_scene_min_feature_size(const xrp_scene* sc, double* out) {
-  if (!sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
-  *out = sc->s->min_feature;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
+    *out = sc->s->min_feature;
+    return XRP_OK;
+  });
 }
 
 int xrp_scene_auto_ds(const xrp_scene* sc, double* out) {
-  if (!sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
-  *out = sc->s->min_feature / 5.0;  // main.go:350-353
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
+    *out = sc->s->min_feature / 5.0;  // main.go:350-353
+    return XRP_OK;
+  });
 }
 
 int xrp_scene_clip_box(const xrp_scene* sc, double lo[3], double hi[3],
                        int* empty) {
-  if (!sc || !lo || !hi || !empty) return fail(XRP_ERR_NULL, "NULL argument");
-  const auto& b = sc->s->world_box;
-  *empty = b.empty() ? 1 : 0;
-  for (int k = 0; k < 3; ++k) {
-    lo[k] = b.lo[k];
-    hi[k] = b.hi[k];
-  }
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!sc || !lo || !hi || !empty) return fail(XRP_ERR_NULL, "NULL argument");
+    const auto& b = sc->s->world_box;
+    *empty = b.empty() ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = b.lo[k];
+      hi[k] = b.hi[k];
+    }
+    return XRP_OK;
+  });
 }
 
 int xrp_scene_layout(const xrp_scene* sc, int* nodes, int* deformations,
                      int* has_voxels) {
-  if (!sc) return fail(XRP_ERR_NULL, "scene is NULL");
-  if (nodes) *nodes = (int)sc->s->nodes.size();
-  if (deformations) *deformations = (int)sc->s->ddefs.size();
-  if (has_voxels) *has_voxels = sc->s->has_voxels ? 1 : 0;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!sc) return fail(XRP_ERR_NULL, "scene is NULL");
+    if (nodes) *nodes = (int)sc->s->nodes.size();
+    if (deformations) *deformations = (int)sc->s->ddefs.size();
+    if (has_voxels) *has_voxels = sc->s->has_voxels ? 1 : 0;
+    return XRP_OK;
+  });
 }
 
 int xrp_camera_from_angles(double az, double polar, double R, double eye[3],
                            double cam[16]) {
-  if (!eye || !cam) return fail(XRP_ERR_NULL, "NULL argument");
-  xrp::camera_from_angles(az, polar, R, eye, cam);
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!eye || !cam) return fail(XRP_ERR_NULL, "NULL argument");
+    xrp::camera_from_angles(az, polar, R, eye, cam);
+    return XRP_OK;
+  });
 }
 
 int xrp_sample_count(double smin, double smax, double ds, int64_t* count) {
-  if (!count) return fail(XRP_ERR_NULL, "count is NULL");
-  if (!(ds > 0.0) || !std::isfinite(ds) || !std::isfinite(smin) ||
-      !std::isfinite(smax))
-    return fail(XRP_ERR_ARG, "ds must be positive and bounds finite");
-  int64_t n = 0;
-  for (double s = smin; s < smax; s += ds) ++n;
-  *count = n;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!count) return fail(XRP_ERR_NULL, "count is NULL");
+    if (!(ds > 0.0) || !std::isfinite(ds) || !std::isfinite(smin) ||
+        !std::isfinite(smax))
+      return fail(XRP_ERR_ARG, "ds must be positive and bounds finite");
+    int64_t n = 0;
+    for (double s = smin; s < smax; s += ds) ++n;
+    *count = n;
+    return XRP_OK;
+  });
 }
 
 }  // extern "C"
@@ -797,19 +869,21 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   if (st) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    std::memset(st, 0, sizeof *st);
-    st->rays = (uint64_t)nv * res * res;
-    st->samples = hst[ST_SAMPLES];
-    st->reevaluated = hst[ST_REEVAL];
-    st->audit_mismatch = hst[ST_AUDIT_MISS];
-    st->fine_samples = hst[ST_FINE];
-    st->staged_chunks = hst[ST_STAGED_CHUNKS];
-    st->staged_voxels = hst[ST_STAGED_VOXELS];
-    st->fallback_chunks = hst[ST_FALLBACK_CHUNKS];
-    st->skipped_samples = hst[ST_SKIPPED];
-    st->kernel_ms = ms;
-    st->position_error = delta;
-    st->precision_used = precision;
+    xrp_render_stats o;
+    std::memset(&o, 0, sizeof o);
+    o.rays = (uint64_t)nv * res * res;
+    o.samples = hst[ST_SAMPLES];
+    o.reevaluated = hst[ST_REEVAL];
+    o.audit_mismatch = hst[ST_AUDIT_MISS];
+    o.fine_samples = hst[ST_FINE];
+    o.staged_chunks = hst[ST_STAGED_CHUNKS];
+    o.staged_voxels = hst[ST_STAGED_VOXELS];
+    o.fallback_chunks = hst[ST_FALLBACK_CHUNKS];
+    o.skipped_samples = hst[ST_SKIPPED];
+    o.kernel_ms = ms;
+    o.position_error = delta;
+    o.precision_used = precision;
+    write_out(st, o);
   }
   return XRP_OK;
 }
@@ -875,136 +949,169 @@ extern "C" {
 int xrp_render_device(xrp_context* ctx, xrp_scene* sc,
                       const xrp_render_params* p, const xrp_view_angle* views,
                       int n_views, void* out_device, xrp_render_stats* st) {
-  if (!ctx || !sc || !views || !out_device)
-    return fail(XRP_ERR_NULL, "NULL argument");
-  int rc = check_params(p, n_views);
-  if (rc) return rc;
-  std::lock_guard<std::mutex> g(ctx->mu);
-  RenderJob job = make_job(p, views, n_views);
-  return render_core(ctx, sc, job, out_device, st);
+  return guarded([&]() -> int {
+    if (!ctx || !sc || !views || !out_device)
+      return fail(XRP_ERR_NULL, "NULL argument");
+    int rc = check_params(p, n_views);
+    if (!rc) rc = check_out(st, "stats");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    RenderJob job = make_job(p, views, n_views);
+    return render_core(ctx, sc, job, out_device, st);
+  });
 }
 
 int xrp_render(xrp_context* ctx, xrp_scene* sc, const xrp_render_params* p,
                const xrp_view_angle* views, int n_views, void* out,
                xrp_render_stats* st) {
-  if (!ctx || !sc || !views || !out) return fail(XRP_ERR_NULL, "NULL argument");
-  int rc = check_params(p, n_views);
-  if (rc) return rc;
-  std::lock_guard<std::mutex> g(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  size_t bytes = (size_t)n_views * p->resolution * p->resolution *
-                 pixel_bytes(p->output_format);
-  HIPCHK(ctx->out.ensure(bytes));
-  RenderJob job = make_job(p, views, n_views);
-  rc = render_core(ctx, sc, job, ctx->out.p, st);
-  if (rc) return rc;
-  HIPCHK(hipMemcpy(out, ctx->out.p, bytes, hipMemcpyDeviceToHost));
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !sc || !views || !out) return fail(XRP_ERR_NULL, "NULL argument");
+    int rc = check_params(p, n_views);
+    if (!rc) rc = check_out(st, "stats");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    size_t bytes = (size_t)n_views * p->resolution * p->resolution *
+                   pixel_bytes(p->output_format);
+    HIPCHK(ctx->out.ensure(bytes));
+    RenderJob job = make_job(p, views, n_views);
+    rc = render_core(ctx, sc, job, ctx->out.p, st);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(out, ctx->out.p, bytes, hipMemcpyDeviceToHost));
+    return XRP_OK;
+  });
 }
 
 int xrp_export_volume(xrp_context* ctx, xrp_scene* sc, int res,
                       double density_multiplier, uint8_t* out) {
-  using namespace xrp;
-  if (!ctx || !sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
-  if (res <= 0 || res > 2048) return fail(XRP_ERR_DIMS, "res must be in 1..2048");
-  std::lock_guard<std::mutex> g(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  Scene& s = *sc->s;
-  std::vector<void*> vptr;
-  if (s.has_voxels) {
-    int rc = upload_voxels(ctx, sc, vptr);
-    if (rc) return rc;
-  }
-  size_t vi = 0;
-  for (auto& n : s.nodes)
-    if (n.kind == K_VOXEL) n.vox = (const float*)vptr[vi++];
-  const size_t n = (size_t)res * res * res;
-  HIPCHK(ctx->nodes.ensure(s.nodes.size() * sizeof(DevNode)));
-  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, s.ddefs.size()) * sizeof(DevDeform)));
-  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
-  DevBuf dens, bytes;
-  HIPCHK(dens.ensure(n * sizeof(double)));
-  hipError_t he = bytes.ensure(n);
-  if (he != hipSuccess) {
-    dens.release();
-    return fail(XRP_ERR_ALLOC, hipGetErrorString(he));
-  }
-  MarchParams mp;
-  std::memset(&mp, 0, sizeof mp);
-  mp.nodes = (const DevNode*)ctx->nodes.p;
-  mp.defs = (const DevDeform*)ctx->defs.p;
-  mp.n_defs = (int)s.ddefs.size();
-  mp.dm = density_multiplier;
-  he = hipMemcpyAsync(ctx->nodes.p, s.nodes.data(), s.nodes.size() * sizeof(DevNode),
-                      hipMemcpyHostToDevice, ctx->stream);
-  if (he == hipSuccess && !s.ddefs.empty())
-    he = hipMemcpyAsync(ctx->defs.p, s.ddefs.data(), s.ddefs.size() * sizeof(DevDeform),
+  return guarded([&]() -> int {
+    using namespace xrp;
+    if (!ctx || !sc || !out) return fail(XRP_ERR_NULL, "NULL argument");
+    if (res <= 0 || res > 2048) return fail(XRP_ERR_DIMS, "res must be in 1..2048");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    Scene& s = *sc->s;
+    std::vector<void*> vptr;
+    if (s.has_voxels) {
+      int rc = upload_voxels(ctx, sc, vptr);
+      if (rc) return rc;
+    }
+    size_t vi = 0;
+    for (auto& n : s.nodes)
+      if (n.kind == K_VOXEL) n.vox = (const float*)vptr[vi++];
+    const size_t n = (size_t)res * res * res;
+    HIPCHK(ctx->nodes.ensure(s.nodes.size() * sizeof(DevNode)));
+    HIPCHK(ctx->defs.ensure(std::max<size_t>(1, s.ddefs.size()) * sizeof(DevDeform)));
+    HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
+    DevBuf dens, bytes;
+    HIPCHK(dens.ensure(n * sizeof(double)));
+    hipError_t he = bytes.ensure(n);
+    if (he != hipSuccess) {
+      dens.release();
+      return fail(XRP_ERR_ALLOC, hipGetErrorString(he));
+    }
+    MarchParams mp;
+    std::memset(&mp, 0, sizeof mp);
+    mp.nodes = (const DevNode*)ctx->nodes.p;
+    mp.defs = (const DevDeform*)ctx->defs.p;
+    mp.n_defs = (int)s.ddefs.size();
+    mp.dm = density_multiplier;
+    he = hipMemcpyAsync(ctx->nodes.p, s.nodes.data(), s.nodes.size() * sizeof(DevNode),
                         hipMemcpyHostToDevice, ctx->stream);
-  if (he == hipSuccess)
-    he = hipMemsetAsync(ctx->stats.p, 0, sizeof(unsigned long long), ctx->stream);
-  if (he == hipSuccess)
-    he = (hipError_t)launch_volume(mp, res, (double*)dens.p, (unsigned long long*)ctx->stats.p,
-                                   (unsigned char*)bytes.p, ctx->stream);
-  if (he == hipSuccess)
-    he = hipMemcpyAsync(out, bytes.p, n, hipMemcpyDeviceToHost, ctx->stream);
-  if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-  dens.release();
-  bytes.release();
-  if (he != hipSuccess)
-    return fail(XRP_ERR_DEVICE, std::string("volume export failed: ") + hipGetErrorString(he));
-  return XRP_OK;
+    if (he == hipSuccess && !s.ddefs.empty())
+      he = hipMemcpyAsync(ctx->defs.p, s.ddefs.data(), s.ddefs.size() * sizeof(DevDeform),
+                          hipMemcpyHostToDevice, ctx->stream);
+    if (he == hipSuccess)
+      he = hipMemsetAsync(ctx->stats.p, 0, sizeof(unsigned long long), ctx->stream);
+    if (he == hipSuccess)
+      he = (hipError_t)launch_volume(mp, res, (double*)dens.p, (unsigned long long*)ctx->stats.p,
+                                     (unsigned char*)bytes.p, ctx->stream);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(out, bytes.p, n, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    dens.release();
+    bytes.release();
+    if (he != hipSuccess)
+      return fail(XRP_ERR_DEVICE, std::string("volume export failed: ") + hipGetErrorString(he));
+    return XRP_OK;
+  });
 }
 
 int xrp_render_volume_projections(xrp_context* ctx, const float* volume, int nx,
                                   int ny, int nz, const xrp_camera* cams,
                                   int n_cams, int image_res, double ds,
                                   double flat_field, float* out) {
-  // cuda_backend.cu:95-100 argument checks, same codes.
-  if (!ctx || !volume || !cams || !out) return fail(XRP_ERR_NULL, "NULL argument");
-  if (nx <= 0 || ny <= 0 || nz <= 0 || image_res <= 0 || n_cams <= 0)
-    return fail(XRP_ERR_DIMS, "dimensions must be positive");
-  if (!(ds > 0.0) || !std::isfinite(ds)) return fail(XRP_ERR_ARG, "ds must be positive");
-  xrp_scene* sc = nullptr;
-  int rc = xrp_scene_create_voxels(volume, nx, ny, nz, nullptr, XRP_NUMBERS_JSON, &sc);
-  if (rc) return rc;
-  std::lock_guard<std::mutex> g(ctx->mu);
-  rc = XRP_OK;
-  size_t img = (size_t)image_res * image_res;
-  hipError_t he = hipSetDevice(ctx->device);
-  if (he == hipSuccess) he = ctx->out.ensure(img * n_cams * sizeof(float));
-  if (he != hipSuccess) rc = fail(XRP_ERR_DEVICE, hipGetErrorString(he));
-  // Consecutive cameras sharing (fov, R) go into one launch.
-  for (int c0 = 0; c0 < n_cams && rc == XRP_OK;) {
-    int c1 = c0 + 1;
-    while (c1 < n_cams && cams[c1].fov_deg == cams[c0].fov_deg &&
-           cams[c1].R == cams[c0].R)
-      ++c1;
-    RenderJob job;
-    job.res = image_res;
-    job.ds = ds;
-    job.R = cams[c0].R;
-    job.fov = cams[c0].fov_deg;
-    job.flat_field = flat_field;
-    job.dm = 1.0;
-    job.integrator = XRP_INTEGRATE_SIMPLE;
-    job.precision = XRP_PRECISION_FP64;
-    job.audit = 0;
-    for (int c = c0; c < c1; ++c) {
-      xrp::DevView dv;
-      std::memset(&dv, 0, sizeof dv);
-      std::memcpy(dv.eye, cams[c].eye, sizeof dv.eye);
-      std::memcpy(dv.cam, cams[c].cam2world, sizeof dv.cam);
-      job.views.push_back(dv);
+  return guarded([&]() -> int {
+    // cuda_backend.cu:95-100 argument checks keep their codes (1 NULL, 2
+    // dimensions); the rest are the limits xrp_render applies.
+    if (!ctx || !volume || !cams || !out) return fail(XRP_ERR_NULL, "NULL argument");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || image_res <= 0 || n_cams <= 0)
+      return fail(XRP_ERR_DIMS, "dimensions must be positive");
+    if (image_res > 32768) return fail(XRP_ERR_DIMS, "image_res above 32768");
+    if ((double)nx * ny * nz > kMaxVoxels)
+      return fail(XRP_ERR_DIMS, "volume larger than 2^33 voxels");
+    if (!(ds > 0.0) || !std::isfinite(ds)) return fail(XRP_ERR_ARG, "ds must be positive");
+    if (2 * xrp::kCubeHalfDiagonal / ds > 5e7)
+      return fail(XRP_ERR_ARG, "ds too small: more than 5e7 samples per ray");
+    if (!std::isfinite(flat_field)) return fail(XRP_ERR_ARG, "flat_field must be finite");
+    for (int c = 0; c < n_cams; ++c) {
+      bool ok = std::isfinite(cams[c].fov_deg) && std::isfinite(cams[c].R);
+      for (int k = 0; k < 3; ++k) ok = ok && std::isfinite(cams[c].eye[k]);
+      for (int k = 0; k < 16; ++k) ok = ok && std::isfinite(cams[c].cam2world[k]);
+      if (!ok)
+        return fail(XRP_ERR_ARG, "camera " + std::to_string(c) +
+                                     ": eye, cam2world, fov_deg and R must be finite");
     }
-    rc = render_core(ctx, sc, job, (void*)((float*)ctx->out.p + img * c0), nullptr);
-    c0 = c1;
-  }
-  if (rc == XRP_OK) {
-    he = hipMemcpy(out, ctx->out.p, img * n_cams * sizeof(float), hipMemcpyDeviceToHost);
-    if (he != hipSuccess) rc = fail(XRP_ERR_DEVICE, hipGetErrorString(he));
-  }
-  xrp_scene_destroy(sc);
-  return rc;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    // The volume's scene, with its device copy and brick copies, stays on
+    // the context while the caller passes the same dimensions and values.
+    const size_t nvox = (size_t)nx * ny * nz;
+    xrp_scene*& sc = ctx->vol_scene;
+    if (sc) {
+      const xrp::HNode& r = sc->s->root;
+      if (r.nx != nx || r.ny != ny || r.nz != nz ||
+          std::memcmp(r.vox->data(), volume, nvox * sizeof(float)) != 0) {
+        xrp_scene_destroy(sc);
+        sc = nullptr;
+      }
+    }
+    if (!sc) {
+      int rc = xrp_scene_create_voxels(volume, nx, ny, nz, nullptr, XRP_NUMBERS_JSON, &sc);
+      if (rc) return rc;
+    }
+    const size_t img = (size_t)image_res * image_res;
+    HIPCHK(ctx->out.ensure(img * n_cams * sizeof(float)));
+    // Consecutive cameras sharing (fov, R) go into one launch.
+    for (int c0 = 0; c0 < n_cams;) {
+      int c1 = c0 + 1;
+      while (c1 < n_cams && c1 - c0 < 65535 && cams[c1].fov_deg == cams[c0].fov_deg &&
+             cams[c1].R == cams[c0].R)
+        ++c1;
+      RenderJob job;
+      job.res = image_res;
+      job.ds = ds;
+      job.R = cams[c0].R;
+      job.fov = cams[c0].fov_deg;
+      job.flat_field = flat_field;
+      job.dm = 1.0;
+      job.integrator = XRP_INTEGRATE_SIMPLE;
+      job.precision = XRP_PRECISION_AUTO;  // the voxel kernel
+      job.audit = 0;
+      for (int c = c0; c < c1; ++c) {
+        xrp::DevView dv;
+        std::memset(&dv, 0, sizeof dv);
+        std::memcpy(dv.eye, cams[c].eye, sizeof dv.eye);
+        std::memcpy(dv.cam, cams[c].cam2world, sizeof dv.cam);
+        job.views.push_back(dv);
+      }
+      int rc = render_core(ctx, sc, job, (void*)((float*)ctx->out.p + img * c0), nullptr);
+      if (rc) return rc;
+      c0 = c1;
+    }
+    HIPCHK(hipMemcpy(out, ctx->out.p, img * n_cams * sizeof(float), hipMemcpyDeviceToHost));
+    return XRP_OK;
+  });
 }
 
 }  // extern "C"
@@ -1040,7 +1147,7 @@ struct xrp_batch {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (DevBuf* b : {&nodes, &defs, &views, &tables, &jobs, &tile_start, &counters, &t_first,
-                      &t_done, &stats, &out})
+                      &t_done, &stats, &out, &sink})
       b->release();
   }
 };
@@ -1186,15 +1293,17 @@ int batch_run_locked(xrp_batch* b, xrp_batch_stats* st) {
           done >= start && b->ticks_per_ms > 0 ? (double)(done - start) / b->ticks_per_ms : 0.0;
     }
   if (st) {
-    std::memset(st, 0, sizeof *st);
-    st->kernel_ms = ms;
+    xrp_batch_stats o;
+    std::memset(&o, 0, sizeof o);
+    o.kernel_ms = ms;
     for (int it = 0; it < b->n_items; ++it) {
-      st->rays += (uint64_t)b->n_views[it] * b->res[it] * b->res[it];
-      st->samples += hst[(size_t)it * ST_COUNT + ST_SAMPLES];
-      st->reevaluated += hst[(size_t)it * ST_COUNT + ST_REEVAL];
+      o.rays += (uint64_t)b->n_views[it] * b->res[it] * b->res[it];
+      o.samples += hst[(size_t)it * ST_COUNT + ST_SAMPLES];
+      o.reevaluated += hst[(size_t)it * ST_COUNT + ST_REEVAL];
     }
-    for (auto& g : b->groups) st->tiles += g.n_tiles;
-    st->launches = (int)b->groups.size();
+    for (auto& g : b->groups) o.tiles += g.n_tiles;
+    o.launches = (int)b->groups.size();
+    write_out(st, o);
   }
   return XRP_OK;
 }
@@ -1205,222 +1314,239 @@ extern "C" {
 
 int xrp_batch_create(xrp_context* ctx, const xrp_batch_item* items, int n_items,
                      xrp_batch** out) {
-  using namespace xrp;
-  if (!ctx || !items || !out) return fail(XRP_ERR_NULL, "NULL argument");
-  *out = nullptr;
-  if (n_items <= 0) return fail(XRP_ERR_DIMS, "n_items must be positive");
-  for (int it = 0; it < n_items; ++it) {
-    if (!items[it].scene || !items[it].views)
-      return fail(XRP_ERR_NULL, "item " + std::to_string(it) + ": NULL scene or views");
-    int rc = check_params(&items[it].params, items[it].n_views);
-    if (rc) return fail(rc, "item " + std::to_string(it) + ": " + g_error);
-    if (items[it].params.output_format != XRP_OUTPUT_F32)
-      return fail(XRP_ERR_ARG, "item " + std::to_string(it) + ": batches write float32 images");
-  }
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  std::unique_ptr<xrp_batch> b(new xrp_batch());
-  b->ctx = ctx;
-  b->n_items = n_items;
-  hipDeviceProp_t prop;
-  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
-  b->n_cu = prop.multiProcessorCount;
-  int khz = 0;
-  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
-  b->ticks_per_ms = (double)khz;
+  return guarded([&]() -> int {
+    using namespace xrp;
+    if (!ctx || !items || !out) return fail(XRP_ERR_NULL, "NULL argument");
+    *out = nullptr;
+    if (n_items <= 0) return fail(XRP_ERR_DIMS, "n_items must be positive");
+    for (int it = 0; it < n_items; ++it) {
+      if (!items[it].scene || !items[it].views)
+        return fail(XRP_ERR_NULL, "item " + std::to_string(it) + ": NULL scene or views");
+      int rc = check_params(&items[it].params, items[it].n_views);
+      if (rc) return fail(rc, "item " + std::to_string(it) + ": " + g_error);
+      if (items[it].params.output_format != XRP_OUTPUT_F32)
+        return fail(XRP_ERR_ARG, "item " + std::to_string(it) + ": batches write float32 images");
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<xrp_batch> b(new xrp_batch());
+    b->ctx = ctx;
+    b->n_items = n_items;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+    b->n_cu = prop.multiProcessorCount;
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    b->ticks_per_ms = (double)khz;
 
-  // Plan every item; concatenate nodes, deformations, views and tables.
-  std::vector<DevNode> all_nodes;
-  std::vector<DevDeform> all_defs;
-  std::vector<DevView> all_views;
-  std::vector<double> all_tables;
-  std::vector<size_t> node_off, def_off, view_off, table_off;
-  size_t out_floats = 0;
-  std::string sig = std::string(prop.gcnArchName) + "/" + std::to_string(b->n_cu);
-  for (int it = 0; it < n_items; ++it) {
-    xrp_scene* sc = const_cast<xrp_scene*>(items[it].scene);
-    RenderJob job = make_job(&items[it].params, items[it].views, items[it].n_views);
-    LaunchPlan pl;
-    int rc = plan_job(ctx, sc, job, pl, /*allow_voxel_kernel=*/false);
+    // Plan every item; concatenate nodes, deformations, views and tables.
+    std::vector<DevNode> all_nodes;
+    std::vector<DevDeform> all_defs;
+    std::vector<DevView> all_views;
+    std::vector<double> all_tables;
+    std::vector<size_t> node_off, def_off, view_off, table_off;
+    size_t out_floats = 0;
+    std::string sig = std::string(prop.gcnArchName) + "/" + std::to_string(b->n_cu);
+    for (int it = 0; it < n_items; ++it) {
+      xrp_scene* sc = const_cast<xrp_scene*>(items[it].scene);
+      RenderJob job = make_job(&items[it].params, items[it].views, items[it].n_views);
+      LaunchPlan pl;
+      int rc = plan_job(ctx, sc, job, pl, /*allow_voxel_kernel=*/false);
+      if (rc) return rc;
+      const Scene& s = *sc->s;
+      node_off.push_back(all_nodes.size());
+      def_off.push_back(all_defs.size());
+      view_off.push_back(all_views.size());
+      table_off.push_back(all_tables.size());
+      all_nodes.insert(all_nodes.end(), pl.nodes.begin(), pl.nodes.end());
+      all_defs.insert(all_defs.end(), s.ddefs.begin(), s.ddefs.end());
+      all_views.insert(all_views.end(), job.views.begin(), job.views.end());
+      all_tables.insert(all_tables.end(), pl.table.begin(), pl.table.end());
+      b->res.push_back(job.res);
+      b->n_views.push_back((int)job.views.size());
+      b->out_off.push_back(out_floats);
+      out_floats += (size_t)job.views.size() * job.res * job.res;
+      b->mps.push_back(pl.mp);
+      b->prec.push_back(pl.precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED);
+      b->integ.push_back(job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
+                                                                     : INT_SIMPLE);
+      // work per tile ~ nodes x samples in the clip box (x2 for FP64)
+      double frac = 1.0;
+      if (pl.mp.clip) {
+        double ext = 0;
+        for (int k = 0; k < 3; ++k) ext += (pl.mp.hi[k] - pl.mp.lo[k]) * (pl.mp.hi[k] - pl.mp.lo[k]);
+        frac = std::min(1.0, std::sqrt(ext) / (2 * kCubeHalfDiagonal));
+      }
+      b->cost.push_back((double)pl.nodes.size() * pl.mp.K * frac *
+                        (b->prec.back() == PREC_FP64 ? 2.0 : 1.0));
+      sig += ";" + std::to_string(pl.nodes.size()) + "," + std::to_string(job.res) + "," +
+             std::to_string(job.views.size()) + "," + std::to_string(pl.mp.K) + "," +
+             std::to_string(b->prec.back()) + "," + std::to_string(b->integ.back());
+    }
+    b->signature = sig;
+
+    // Groups by (precision, integrator), items by cost, most expensive first.
+    for (int it = 0; it < n_items; ++it) {
+      auto g = std::find_if(b->groups.begin(), b->groups.end(), [&](const xrp_batch::Group& x) {
+        return x.prec == b->prec[it] && x.integ == b->integ[it];
+      });
+      if (g == b->groups.end()) {
+        b->groups.push_back(xrp_batch::Group());
+        g = b->groups.end() - 1;
+        g->prec = b->prec[it];
+        g->integ = b->integ[it];
+      }
+      g->items.push_back(it);
+    }
+    int first = 0;
+    for (auto& g : b->groups) {
+      std::stable_sort(g.items.begin(), g.items.end(),
+                       [&](int a, int c) { return b->cost[a] > b->cost[c]; });
+      g.first = first;
+      first += (int)g.items.size();
+    }
+
+    // Device copies.
+    HIPCHK(b->nodes.ensure(std::max<size_t>(1, all_nodes.size()) * sizeof(DevNode)));
+    HIPCHK(b->defs.ensure(std::max<size_t>(1, all_defs.size()) * sizeof(DevDeform)));
+    HIPCHK(b->views.ensure(all_views.size() * sizeof(DevView)));
+    HIPCHK(b->tables.ensure(std::max<size_t>(1, all_tables.size()) * sizeof(double)));
+    HIPCHK(b->jobs.ensure((size_t)n_items * sizeof(MarchParams)));
+    HIPCHK(b->counters.ensure(b->groups.size() * sizeof(uint32_t)));
+    HIPCHK(b->t_first.ensure((size_t)n_items * sizeof(unsigned long long)));
+    HIPCHK(b->t_done.ensure((size_t)n_items * sizeof(unsigned long long)));
+    HIPCHK(b->sink.ensure(kBatchSinkBytes));
+    HIPCHK(b->stats.ensure((size_t)n_items * ST_COUNT * sizeof(unsigned long long)));
+    HIPCHK(b->out.ensure(out_floats * sizeof(float)));
+    if (!all_nodes.empty())
+      HIPCHK(hipMemcpy(b->nodes.p, all_nodes.data(), all_nodes.size() * sizeof(DevNode),
+                       hipMemcpyHostToDevice));
+    if (!all_defs.empty())
+      HIPCHK(hipMemcpy(b->defs.p, all_defs.data(), all_defs.size() * sizeof(DevDeform),
+                       hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->views.p, all_views.data(), all_views.size() * sizeof(DevView),
+                     hipMemcpyHostToDevice));
+    if (!all_tables.empty())
+      HIPCHK(hipMemcpy(b->tables.p, all_tables.data(), all_tables.size() * sizeof(double),
+                       hipMemcpyHostToDevice));
+    std::vector<MarchParams> jobs(n_items);
+    for (auto& g : b->groups)
+      for (size_t k = 0; k < g.items.size(); ++k) {
+        const int it = g.items[k];
+        MarchParams mp = b->mps[it];
+        mp.nodes = (const DevNode*)b->nodes.p + node_off[it];
+        mp.defs = (const DevDeform*)b->defs.p + def_off[it];
+        mp.views = (const DevView*)b->views.p + view_off[it];
+        mp.s_table = (const double*)b->tables.p + table_off[it];
+        mp.stats = (unsigned long long*)b->stats.p + (size_t)it * ST_COUNT;
+        mp.out = (float*)b->out.p + b->out_off[it];
+        mp.view_offset = 0;
+        mp.n_views = b->n_views[it];
+        b->mps[it] = mp;
+        jobs[g.first + k] = mp;
+      }
+    HIPCHK(hipMemcpy(b->jobs.p, jobs.data(), jobs.size() * sizeof(MarchParams),
+                     hipMemcpyHostToDevice));
+
+    xrp_tile_shape tuned;
+    if (lookup_tuned(b->signature, tuned)) b->shape = tuned;
+    int rc = apply_shape(b.get());
     if (rc) return rc;
-    const Scene& s = *sc->s;
-    node_off.push_back(all_nodes.size());
-    def_off.push_back(all_defs.size());
-    view_off.push_back(all_views.size());
-    table_off.push_back(all_tables.size());
-    all_nodes.insert(all_nodes.end(), pl.nodes.begin(), pl.nodes.end());
-    all_defs.insert(all_defs.end(), s.ddefs.begin(), s.ddefs.end());
-    all_views.insert(all_views.end(), job.views.begin(), job.views.end());
-    all_tables.insert(all_tables.end(), pl.table.begin(), pl.table.end());
-    b->res.push_back(job.res);
-    b->n_views.push_back((int)job.views.size());
-    b->out_off.push_back(out_floats);
-    out_floats += (size_t)job.views.size() * job.res * job.res;
-    b->mps.push_back(pl.mp);
-    b->prec.push_back(pl.precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED);
-    b->integ.push_back(job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
-                                                                   : INT_SIMPLE);
-    // work per tile ~ nodes x samples in the clip box (x2 for FP64)
-    double frac = 1.0;
-    if (pl.mp.clip) {
-      double ext = 0;
-      for (int k = 0; k < 3; ++k) ext += (pl.mp.hi[k] - pl.mp.lo[k]) * (pl.mp.hi[k] - pl.mp.lo[k]);
-      frac = std::min(1.0, std::sqrt(ext) / (2 * kCubeHalfDiagonal));
-    }
-    b->cost.push_back((double)pl.nodes.size() * pl.mp.K * frac *
-                      (b->prec.back() == PREC_FP64 ? 2.0 : 1.0));
-    sig += ";" + std::to_string(pl.nodes.size()) + "," + std::to_string(job.res) + "," +
-           std::to_string(job.views.size()) + "," + std::to_string(pl.mp.K) + "," +
-           std::to_string(b->prec.back()) + "," + std::to_string(b->integ.back());
-  }
-  b->signature = sig;
-
-  // Groups by (precision, integrator), items by cost, most expensive first.
-  for (int it = 0; it < n_items; ++it) {
-    auto g = std::find_if(b->groups.begin(), b->groups.end(), [&](const xrp_batch::Group& x) {
-      return x.prec == b->prec[it] && x.integ == b->integ[it];
-    });
-    if (g == b->groups.end()) {
-      b->groups.push_back(xrp_batch::Group());
-      g = b->groups.end() - 1;
-      g->prec = b->prec[it];
-      g->integ = b->integ[it];
-    }
-    g->items.push_back(it);
-  }
-  int first = 0;
-  for (auto& g : b->groups) {
-    std::stable_sort(g.items.begin(), g.items.end(),
-                     [&](int a, int c) { return b->cost[a] > b->cost[c]; });
-    g.first = first;
-    first += (int)g.items.size();
-  }
-
-  // Device copies.
-  HIPCHK(b->nodes.ensure(std::max<size_t>(1, all_nodes.size()) * sizeof(DevNode)));
-  HIPCHK(b->defs.ensure(std::max<size_t>(1, all_defs.size()) * sizeof(DevDeform)));
-  HIPCHK(b->views.ensure(all_views.size() * sizeof(DevView)));
-  HIPCHK(b->tables.ensure(std::max<size_t>(1, all_tables.size()) * sizeof(double)));
-  HIPCHK(b->jobs.ensure((size_t)n_items * sizeof(MarchParams)));
-  HIPCHK(b->counters.ensure(b->groups.size() * sizeof(uint32_t)));
-  HIPCHK(b->t_first.ensure((size_t)n_items * sizeof(unsigned long long)));
-  HIPCHK(b->t_done.ensure((size_t)n_items * sizeof(unsigned long long)));
-  HIPCHK(b->sink.ensure(kBatchSinkBytes));
-  HIPCHK(b->stats.ensure((size_t)n_items * ST_COUNT * sizeof(unsigned long long)));
-  HIPCHK(b->out.ensure(out_floats * sizeof(float)));
-  if (!all_nodes.empty())
-    HIPCHK(hipMemcpy(b->nodes.p, all_nodes.data(), all_nodes.size() * sizeof(DevNode),
-                     hipMemcpyHostToDevice));
-  if (!all_defs.empty())
-    HIPCHK(hipMemcpy(b->defs.p, all_defs.data(), all_defs.size() * sizeof(DevDeform),
-                     hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b->views.p, all_views.data(), all_views.size() * sizeof(DevView),
-                   hipMemcpyHostToDevice));
-  if (!all_tables.empty())
-    HIPCHK(hipMemcpy(b->tables.p, all_tables.data(), all_tables.size() * sizeof(double),
-                     hipMemcpyHostToDevice));
-  std::vector<MarchParams> jobs(n_items);
-  for (auto& g : b->groups)
-    for (size_t k = 0; k < g.items.size(); ++k) {
-      const int it = g.items[k];
-      MarchParams mp = b->mps[it];
-      mp.nodes = (const DevNode*)b->nodes.p + node_off[it];
-      mp.defs = (const DevDeform*)b->defs.p + def_off[it];
-      mp.views = (const DevView*)b->views.p + view_off[it];
-      mp.s_table = (const double*)b->tables.p + table_off[it];
-      mp.stats = (unsigned long long*)b->stats.p + (size_t)it * ST_COUNT;
-      mp.out = (float*)b->out.p + b->out_off[it];
-      mp.view_offset = 0;
-      mp.n_views = b->n_views[it];
-      b->mps[it] = mp;
-      jobs[g.first + k] = mp;
-    }
-  HIPCHK(hipMemcpy(b->jobs.p, jobs.data(), jobs.size() * sizeof(MarchParams),
-                   hipMemcpyHostToDevice));
-
-  xrp_tile_shape tuned;
-  if (lookup_tuned(b->signature, tuned)) b->shape = tuned;
-  int rc = apply_shape(b.get());
-  if (rc) return rc;
-  *out = b.release();
-  return XRP_OK;
+    *out = b.release();
+    return XRP_OK;
+  });
 }
 
 int xrp_batch_set_shape(xrp_batch* b, const xrp_tile_shape* shape) {
-  if (!b || !shape) return fail(XRP_ERR_NULL, "NULL argument");
-  if (!valid_shape(*shape))
-    return fail(XRP_ERR_ARG, "tile shape: 1 <= rows, cols; rows*cols <= 16; "
-                             "1 <= groups_per_cu <= 16");
-  std::lock_guard<std::mutex> lk(b->ctx->mu);
-  HIPCHK(hipSetDevice(b->ctx->device));
-  b->shape = *shape;
-  b->shape.reserved = 0;
-  return apply_shape(b);
+  return guarded([&]() -> int {
+    if (!b || !shape) return fail(XRP_ERR_NULL, "NULL argument");
+    if (!valid_shape(*shape))
+      return fail(XRP_ERR_ARG, "tile shape: 1 <= rows, cols; rows*cols <= 16; "
+                               "1 <= groups_per_cu <= 16");
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    b->shape = *shape;
+    b->shape.reserved = 0;
+    return apply_shape(b);
+  });
 }
 
 int xrp_batch_run(xrp_batch* b, xrp_batch_stats* st) {
-  if (!b) return fail(XRP_ERR_NULL, "NULL batch");
-  std::lock_guard<std::mutex> lk(b->ctx->mu);
-  return batch_run_locked(b, st);
+  return guarded([&]() -> int {
+    if (!b) return fail(XRP_ERR_NULL, "NULL batch");
+    int rc = check_out(st, "stats");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    return batch_run_locked(b, st);
+  });
 }
 
 int xrp_batch_tune(xrp_batch* b, int repeats, xrp_tile_shape* chosen) {
-  if (!b) return fail(XRP_ERR_NULL, "NULL batch");
-  if (repeats < 1) repeats = 1;
-  static const xrp_tile_shape cands[] = {
-      {1, 1, 2, 0}, {1, 1, 4, 0}, {2, 1, 2, 0}, {2, 2, 2, 0}, {4, 1, 2, 0},
-      {4, 2, 2, 0}, {8, 1, 2, 0}, {8, 2, 2, 0}, {4, 4, 2, 0}, {8, 1, 3, 0}};
-  std::lock_guard<std::mutex> lk(b->ctx->mu);
-  HIPCHK(hipSetDevice(b->ctx->device));
-  double best_ms = INFINITY;
-  xrp_tile_shape best = b->shape;
-  for (const auto& c : cands) {
-    b->shape = c;
+  return guarded([&]() -> int {
+    if (!b) return fail(XRP_ERR_NULL, "NULL batch");
+    if (repeats < 1) repeats = 1;
+    static const xrp_tile_shape cands[] = {
+        {1, 1, 2, 0}, {1, 1, 4, 0}, {2, 1, 2, 0}, {2, 2, 2, 0}, {4, 1, 2, 0},
+        {4, 2, 2, 0}, {8, 1, 2, 0}, {8, 2, 2, 0}, {4, 4, 2, 0}, {8, 1, 3, 0}};
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    double best_ms = INFINITY;
+    xrp_tile_shape best = b->shape;
+    for (const auto& c : cands) {
+      b->shape = c;
+      int rc = apply_shape(b);
+      if (rc) return rc;
+      double ms = INFINITY;
+      for (int r = 0; r < repeats; ++r) {
+        xrp_batch_stats st;
+        st.struct_size = sizeof st;
+        rc = batch_run_locked(b, &st);
+        if (rc) return rc;
+        ms = std::min(ms, st.kernel_ms);
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+    b->shape = best;
     int rc = apply_shape(b);
     if (rc) return rc;
-    double ms = INFINITY;
-    for (int r = 0; r < repeats; ++r) {
-      xrp_batch_stats st;
-      rc = batch_run_locked(b, &st);
-      if (rc) return rc;
-      ms = std::min(ms, st.kernel_ms);
-    }
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = c;
-    }
-  }
-  b->shape = best;
-  int rc = apply_shape(b);
-  if (rc) return rc;
-  record_tuned(b->signature, best);
-  if (chosen) *chosen = best;
-  return XRP_OK;
+    record_tuned(b->signature, best);
+    if (chosen) *chosen = best;
+    return XRP_OK;
+  });
 }
 
 int xrp_batch_output(const xrp_batch* b, int item, const float** device_images) {
-  if (!b || !device_images) return fail(XRP_ERR_NULL, "NULL argument");
-  if (item < 0 || item >= b->n_items) return fail(XRP_ERR_ARG, "item out of range");
-  *device_images = (const float*)b->out.p + b->out_off[item];
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!b || !device_images) return fail(XRP_ERR_NULL, "NULL argument");
+    if (item < 0 || item >= b->n_items) return fail(XRP_ERR_ARG, "item out of range");
+    *device_images = (const float*)b->out.p + b->out_off[item];
+    return XRP_OK;
+  });
 }
 
 int xrp_batch_read(xrp_batch* b, int item, float* host_images) {
-  if (!b || !host_images) return fail(XRP_ERR_NULL, "NULL argument");
-  if (item < 0 || item >= b->n_items) return fail(XRP_ERR_ARG, "item out of range");
-  std::lock_guard<std::mutex> lk(b->ctx->mu);
-  HIPCHK(hipSetDevice(b->ctx->device));
-  const size_t n = (size_t)b->n_views[item] * b->res[item] * b->res[item];
-  HIPCHK(hipMemcpy(host_images, (const float*)b->out.p + b->out_off[item], n * sizeof(float),
-                   hipMemcpyDeviceToHost));
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!b || !host_images) return fail(XRP_ERR_NULL, "NULL argument");
+    if (item < 0 || item >= b->n_items) return fail(XRP_ERR_ARG, "item out of range");
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const size_t n = (size_t)b->n_views[item] * b->res[item] * b->res[item];
+    HIPCHK(hipMemcpy(host_images, (const float*)b->out.p + b->out_off[item], n * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    return XRP_OK;
+  });
 }
 
 int xrp_batch_latencies(const xrp_batch* b, double* ms) {
-  if (!b || !ms) return fail(XRP_ERR_NULL, "NULL argument");
-  for (int it = 0; it < b->n_items; ++it)
-    ms[it] = it < (int)b->latency.size() ? b->latency[it] : 0.0;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!b || !ms) return fail(XRP_ERR_NULL, "NULL argument");
+    for (int it = 0; it < b->n_items; ++it)
+      ms[it] = it < (int)b->latency.size() ? b->latency[it] : 0.0;
+    return XRP_OK;
+  });
 }
 
 void xrp_batch_destroy(xrp_batch* b) {
@@ -1528,375 +1654,406 @@ void xrp_train_default_config(xrp_train_config* c) {
 
 int xrp_surrogate_create(xrp_context* ctx, const xrp_scene* scene,
                          const xrp_surrogate_config* cfg, xrp_surrogate** out) {
-  using namespace xrp;
-  if (!ctx || !scene || !cfg || !out) return fail(XRP_ERR_NULL, "NULL argument");
-  *out = nullptr;
-  if (cfg->struct_size < sizeof(xrp_surrogate_config))
-    return fail(XRP_ERR_ARG, "config.struct_size too small");
-  if (cfg->log2_table < 4 || cfg->log2_table > 24)
-    return fail(XRP_ERR_ARG, "log2_table must be in 4..24");
-  if (cfg->n_min < 1 || cfg->n_max < cfg->n_min || cfg->n_max > 65535)
-    return fail(XRP_ERR_ARG, "need 1 <= n_min <= n_max <= 65535");
-  const Scene& s = *scene->s;
-  if (s.has_voxels) return fail(XRP_ERR_UNSUPPORTED, "surrogate of voxel scenes");
-  std::unique_ptr<xrp_surrogate> sr(new xrp_surrogate());
-  sr->ctx = ctx;
-  sr->scene = const_cast<xrp_scene*>(scene);
-  sr->log2_table = cfg->log2_table;
-  SurNet& net = sr->net;
-  std::memset(&net, 0, sizeof net);
-  net.log2_table = cfg->log2_table;
-  const double b = std::exp((std::log((double)cfg->n_max) - std::log((double)cfg->n_min)) /
-                            (kSurLevels - 1));
-  for (int l = 0; l < kSurLevels; ++l)
-    net.res[l] = (int)std::floor((double)cfg->n_min * std::pow(b, (double)l));
-  const bool tess = s.root.kind == K_TESS;
-  net.wrap = cfg->wrap < 0 ? (tess ? 1 : 0) : cfg->wrap;
-  if (net.wrap && !tess) return fail(XRP_ERR_ARG, "wrap needs a tessellated root");
-  double lo[3], ext[3];
-  if (net.wrap) {
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = s.root.ucb[2 * k];
-      ext[k] = s.root.ucb[2 * k + 1] - s.root.ucb[2 * k];
+  return guarded([&]() -> int {
+    using namespace xrp;
+    if (!ctx || !scene || !cfg || !out) return fail(XRP_ERR_NULL, "NULL argument");
+    *out = nullptr;
+    if (cfg->struct_size < sizeof(xrp_surrogate_config))
+      return fail(XRP_ERR_ARG, "config.struct_size too small");
+    if (cfg->log2_table < 4 || cfg->log2_table > 24)
+      return fail(XRP_ERR_ARG, "log2_table must be in 4..24");
+    if (cfg->n_min < 1 || cfg->n_max < cfg->n_min || cfg->n_max > 65535)
+      return fail(XRP_ERR_ARG, "need 1 <= n_min <= n_max <= 65535");
+    const Scene& s = *scene->s;
+    if (s.has_voxels) return fail(XRP_ERR_UNSUPPORTED, "surrogate of voxel scenes");
+    std::unique_ptr<xrp_surrogate> sr(new xrp_surrogate());
+    sr->ctx = ctx;
+    sr->scene = const_cast<xrp_scene*>(scene);
+    sr->log2_table = cfg->log2_table;
+    SurNet& net = sr->net;
+    std::memset(&net, 0, sizeof net);
+    net.log2_table = cfg->log2_table;
+    const double b = std::exp((std::log((double)cfg->n_max) - std::log((double)cfg->n_min)) /
+                              (kSurLevels - 1));
+    for (int l = 0; l < kSurLevels; ++l)
+      net.res[l] = (int)std::floor((double)cfg->n_min * std::pow(b, (double)l));
+    const bool tess = s.root.kind == K_TESS;
+    net.wrap = cfg->wrap < 0 ? (tess ? 1 : 0) : cfg->wrap;
+    if (net.wrap && !tess) return fail(XRP_ERR_ARG, "wrap needs a tessellated root");
+    double lo[3], ext[3];
+    if (net.wrap) {
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = s.root.ucb[2 * k];
+        ext[k] = s.root.ucb[2 * k + 1] - s.root.ucb[2 * k];
+      }
+      for (int k = 0; k < 6; ++k) net.bounds[k] = (float)s.root.bounds[k];
+    } else {
+      const Box3& ob = s.obj_box;
+      if (ob.empty() || !ob.finite())
+        return fail(XRP_ERR_UNSUPPORTED, "surrogate needs a bounded object");
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = ob.lo[k];
+        ext[k] = ob.hi[k] - ob.lo[k];
+      }
     }
-    for (int k = 0; k < 6; ++k) net.bounds[k] = (float)s.root.bounds[k];
-  } else {
-    const Box3& ob = s.obj_box;
-    if (ob.empty() || !ob.finite())
-      return fail(XRP_ERR_UNSUPPORTED, "surrogate needs a bounded object");
     for (int k = 0; k < 3; ++k) {
-      lo[k] = ob.lo[k];
-      ext[k] = ob.hi[k] - ob.lo[k];
+      if (!(ext[k] > 0)) return fail(XRP_ERR_ARG, "degenerate surrogate domain");
+      net.lo[k] = (float)lo[k];
+      net.ext[k] = (float)ext[k];
+      net.inv_ext[k] = (float)(1.0 / ext[k]);
     }
-  }
-  for (int k = 0; k < 3; ++k) {
-    if (!(ext[k] > 0)) return fail(XRP_ERR_ARG, "degenerate surrogate domain");
-    net.lo[k] = (float)lo[k];
-    net.ext[k] = (float)ext[k];
-    net.inv_ext[k] = (float)(1.0 / ext[k]);
-  }
-  net.rho_scale = (float)std::max(1e-6, rho_scale(s.root));
-  sr->count = (int64_t)sur_param_count(cfg->log2_table);
-  sr->table_count = (int64_t)sur_table_floats(cfg->log2_table);
-  std::vector<float> host((size_t)sr->count);
-  init_params(host, cfg->log2_table, cfg->seed);
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  const size_t bytes = (size_t)sr->count * sizeof(float);
-  HIPCHK(sr->params.ensure(bytes));
-  HIPCHK(sr->grads.ensure(bytes));
-  HIPCHK(sr->m.ensure(bytes));
-  HIPCHK(sr->v.ensure(bytes));
-  HIPCHK(sr->tbf16.ensure(((size_t)kSurLevels << cfg->log2_table) * sizeof(uint32_t)));
-  HIPCHK(sr->loss.ensure(sizeof(double)));
-  HIPCHK(hipMemcpy(sr->params.p, host.data(), bytes, hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(sr->m.p, 0, bytes));
-  HIPCHK(hipMemset(sr->v.p, 0, bytes));
-  HIPCHK(sr->scene_nodes.ensure(s.nodes.size() * sizeof(DevNode)));
-  HIPCHK(hipMemcpy(sr->scene_nodes.p, s.nodes.data(), s.nodes.size() * sizeof(DevNode),
-                   hipMemcpyHostToDevice));
-  net.params = (float*)sr->params.p;
-  net.table_bf16 = (uint32_t*)sr->tbf16.p;
-  *out = sr.release();
-  return XRP_OK;
+    net.rho_scale = (float)std::max(1e-6, rho_scale(s.root));
+    sr->count = (int64_t)sur_param_count(cfg->log2_table);
+    sr->table_count = (int64_t)sur_table_floats(cfg->log2_table);
+    std::vector<float> host((size_t)sr->count);
+    init_params(host, cfg->log2_table, cfg->seed);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)sr->count * sizeof(float);
+    HIPCHK(sr->params.ensure(bytes));
+    HIPCHK(sr->grads.ensure(bytes));
+    HIPCHK(sr->m.ensure(bytes));
+    HIPCHK(sr->v.ensure(bytes));
+    HIPCHK(sr->tbf16.ensure(((size_t)kSurLevels << cfg->log2_table) * sizeof(uint32_t)));
+    HIPCHK(sr->loss.ensure(sizeof(double)));
+    HIPCHK(hipMemcpy(sr->params.p, host.data(), bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(sr->m.p, 0, bytes));
+    HIPCHK(hipMemset(sr->v.p, 0, bytes));
+    HIPCHK(sr->scene_nodes.ensure(s.nodes.size() * sizeof(DevNode)));
+    HIPCHK(hipMemcpy(sr->scene_nodes.p, s.nodes.data(), s.nodes.size() * sizeof(DevNode),
+                     hipMemcpyHostToDevice));
+    net.params = (float*)sr->params.p;
+    net.table_bf16 = (uint32_t*)sr->tbf16.p;
+    *out = sr.release();
+    return XRP_OK;
+  });
 }
 
 int64_t xrp_surrogate_param_count(const xrp_surrogate* s) { return s ? s->count : 0; }
 
 int xrp_surrogate_get_params(const xrp_surrogate* s, float* host) {
-  if (!s || !host) return fail(XRP_ERR_NULL, "NULL argument");
-  std::lock_guard<std::mutex> lk(s->ctx->mu);
-  HIPCHK(hipSetDevice(s->ctx->device));
-  HIPCHK(hipMemcpy(host, s->params.p, (size_t)s->count * sizeof(float), hipMemcpyDeviceToHost));
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!s || !host) return fail(XRP_ERR_NULL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    HIPCHK(hipSetDevice(s->ctx->device));
+    HIPCHK(hipMemcpy(host, s->params.p, (size_t)s->count * sizeof(float), hipMemcpyDeviceToHost));
+    return XRP_OK;
+  });
 }
 
 int xrp_surrogate_set_params(xrp_surrogate* s, const float* host) {
-  if (!s || !host) return fail(XRP_ERR_NULL, "NULL argument");
-  std::lock_guard<std::mutex> lk(s->ctx->mu);
-  HIPCHK(hipSetDevice(s->ctx->device));
-  HIPCHK(hipMemcpy(s->params.p, host, (size_t)s->count * sizeof(float), hipMemcpyHostToDevice));
-  s->packed = false;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!s || !host) return fail(XRP_ERR_NULL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    HIPCHK(hipSetDevice(s->ctx->device));
+    HIPCHK(hipMemcpy(s->params.p, host, (size_t)s->count * sizeof(float), hipMemcpyHostToDevice));
+    s->packed = false;
+    return XRP_OK;
+  });
 }
 
 int xrp_surrogate_predict(xrp_surrogate* s, const float* u_host, int64_t n, float* out_host) {
-  using namespace xrp;
-  if (!s || !u_host || !out_host) return fail(XRP_ERR_NULL, "NULL argument");
-  if (n <= 0) return fail(XRP_ERR_DIMS, "n must be positive");
-  std::lock_guard<std::mutex> lk(s->ctx->mu);
-  HIPCHK(hipSetDevice(s->ctx->device));
-  HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
-  HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
-  HIPCHK(hipMemcpy(s->pts.p, u_host, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
-  int e = launch_sur_predict(s->net, (const float*)s->pts.p, n, (float*)s->labels.p,
-                             s->ctx->stream);
-  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-  HIPCHK(hipStreamSynchronize(s->ctx->stream));
-  HIPCHK(hipMemcpy(out_host, s->labels.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
-  return XRP_OK;
+  return guarded([&]() -> int {
+    using namespace xrp;
+    if (!s || !u_host || !out_host) return fail(XRP_ERR_NULL, "NULL argument");
+    if (n <= 0) return fail(XRP_ERR_DIMS, "n must be positive");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    HIPCHK(hipSetDevice(s->ctx->device));
+    HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
+    HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
+    HIPCHK(hipMemcpy(s->pts.p, u_host, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
+    int e = launch_sur_predict(s->net, (const float*)s->pts.p, n, (float*)s->labels.p,
+                               s->ctx->stream);
+    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    HIPCHK(hipMemcpy(out_host, s->labels.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    return XRP_OK;
+  });
 }
 
 int xrp_surrogate_gradients(xrp_surrogate* s, const float* u_host, const float* targets,
                             int64_t n, float* grads_host, double* loss) {
-  using namespace xrp;
-  if (!s || !u_host || !targets || !grads_host) return fail(XRP_ERR_NULL, "NULL argument");
-  if (n <= 0) return fail(XRP_ERR_DIMS, "n must be positive");
-  std::lock_guard<std::mutex> lk(s->ctx->mu);
-  HIPCHK(hipSetDevice(s->ctx->device));
-  HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
-  HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
-  HIPCHK(hipMemcpy(s->pts.p, u_host, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(s->labels.p, targets, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemsetAsync(s->grads.p, 0, (size_t)s->count * sizeof(float), s->ctx->stream));
-  HIPCHK(hipMemsetAsync(s->loss.p, 0, sizeof(double), s->ctx->stream));
-  int e = launch_sur_train(s->net, (const float*)s->pts.p, (const float*)s->labels.p, n,
-                           (float)(1.0 / (double)n), (float*)s->grads.p, (double*)s->loss.p,
-                           s->ctx->stream);
-  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-  HIPCHK(hipStreamSynchronize(s->ctx->stream));
-  HIPCHK(hipMemcpy(grads_host, s->grads.p, (size_t)s->count * sizeof(float),
-                   hipMemcpyDeviceToHost));
-  double l = 0;
-  HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
-  if (loss) *loss = l / (double)n;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    using namespace xrp;
+    if (!s || !u_host || !targets || !grads_host) return fail(XRP_ERR_NULL, "NULL argument");
+    if (n <= 0) return fail(XRP_ERR_DIMS, "n must be positive");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    HIPCHK(hipSetDevice(s->ctx->device));
+    HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
+    HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
+    HIPCHK(hipMemcpy(s->pts.p, u_host, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->labels.p, targets, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(s->grads.p, 0, (size_t)s->count * sizeof(float), s->ctx->stream));
+    HIPCHK(hipMemsetAsync(s->loss.p, 0, sizeof(double), s->ctx->stream));
+    int e = launch_sur_train(s->net, (const float*)s->pts.p, (const float*)s->labels.p, n,
+                             (float)(1.0 / (double)n), (float*)s->grads.p, (double*)s->loss.p,
+                             s->ctx->stream);
+    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    HIPCHK(hipMemcpy(grads_host, s->grads.p, (size_t)s->count * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    double l = 0;
+    HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
+    if (loss) *loss = l / (double)n;
+    return XRP_OK;
+  });
 }
 
 int xrp_surrogate_train(xrp_surrogate* s, const xrp_train_config* cfg, xrp_train_stats* st) {
-  using namespace xrp;
-  if (!s || !cfg) return fail(XRP_ERR_NULL, "NULL argument");
-  if (cfg->struct_size < sizeof(xrp_train_config))
-    return fail(XRP_ERR_ARG, "config.struct_size too small");
-  if (cfg->batch <= 0 || cfg->samples <= 0 || cfg->epochs <= 0)
-    return fail(XRP_ERR_ARG, "batch, samples and epochs must be positive");
-  xrp_context* ctx = s->ctx;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  const int64_t n = cfg->samples;
-  HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
-  HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
-  MarchParams scene;
-  std::memset(&scene, 0, sizeof scene);
-  scene.nodes = (const DevNode*)s->scene_nodes.p;
-  hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
-  HIPCHK(hipEventRecord(e0, ctx->stream));
-  int e = launch_sur_labels(s->net, scene, (uint64_t)(s->adam_step + 1) * 7919ull, n,
-                            (float*)s->pts.p, (float*)s->labels.p, ctx->stream);
-  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-  HIPCHK(hipEventRecord(e1, ctx->stream));
-  HIPCHK(hipEventSynchronize(e1));
-  float label_ms = 0;
-  HIPCHK(hipEventElapsedTime(&label_ms, e0, e1));
+  return guarded([&]() -> int {
+    using namespace xrp;
+    if (!s || !cfg) return fail(XRP_ERR_NULL, "NULL argument");
+    if (cfg->struct_size < sizeof(xrp_train_config))
+      return fail(XRP_ERR_ARG, "config.struct_size too small");
+    if (cfg->batch <= 0 || cfg->samples <= 0 || cfg->epochs <= 0)
+      return fail(XRP_ERR_ARG, "batch, samples and epochs must be positive");
+    int rc0 = check_out(st, "stats");
+    if (rc0) return rc0;
+    xrp_context* ctx = s->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t n = cfg->samples;
+    HIPCHK(s->pts.ensure((size_t)n * 3 * sizeof(float)));
+    HIPCHK(s->labels.ensure((size_t)n * sizeof(float)));
+    MarchParams scene;
+    std::memset(&scene, 0, sizeof scene);
+    scene.nodes = (const DevNode*)s->scene_nodes.p;
+    hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    int e = launch_sur_labels(s->net, scene, (uint64_t)(s->adam_step + 1) * 7919ull, n,
+                              (float*)s->pts.p, (float*)s->labels.p, ctx->stream);
+    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float label_ms = 0;
+    HIPCHK(hipEventElapsedTime(&label_ms, e0, e1));
 
-  double first = NAN, epoch_loss = 0;
-  int steps = 0;
-  HIPCHK(hipEventRecord(e0, ctx->stream));
-  for (int ep = 0; ep < cfg->epochs; ++ep) {
-    HIPCHK(hipMemsetAsync(s->loss.p, 0, sizeof(double), ctx->stream));
-    for (int64_t b0 = 0; b0 < n; b0 += cfg->batch) {
-      const int64_t nb = std::min<int64_t>(cfg->batch, n - b0);
-      HIPCHK(hipMemsetAsync(s->grads.p, 0, (size_t)s->count * sizeof(float), ctx->stream));
-      e = launch_sur_train(s->net, (const float*)s->pts.p + 3 * b0,
-                           (const float*)s->labels.p + b0, nb, (float)(1.0 / (double)nb),
-                           (float*)s->grads.p, (double*)s->loss.p, ctx->stream);
-      if (!e)
-        e = launch_sur_adam((float*)s->params.p, (const float*)s->grads.p, (float*)s->m.p,
-                            (float*)s->v.p, s->count, s->table_count, (float)cfg->lr_table,
-                            (float)cfg->lr_mlp, 0.9f, 0.99f, 1e-15f, ++s->adam_step,
-                            ctx->stream);
-      if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-      ++steps;
-      if (std::isnan(first)) {
-        double l = 0;
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
-        first = l / (double)nb;
+    double first = NAN, epoch_loss = 0;
+    int steps = 0;
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int ep = 0; ep < cfg->epochs; ++ep) {
+      HIPCHK(hipMemsetAsync(s->loss.p, 0, sizeof(double), ctx->stream));
+      for (int64_t b0 = 0; b0 < n; b0 += cfg->batch) {
+        const int64_t nb = std::min<int64_t>(cfg->batch, n - b0);
+        HIPCHK(hipMemsetAsync(s->grads.p, 0, (size_t)s->count * sizeof(float), ctx->stream));
+        e = launch_sur_train(s->net, (const float*)s->pts.p + 3 * b0,
+                             (const float*)s->labels.p + b0, nb, (float)(1.0 / (double)nb),
+                             (float*)s->grads.p, (double*)s->loss.p, ctx->stream);
+        if (!e)
+          e = launch_sur_adam((float*)s->params.p, (const float*)s->grads.p, (float*)s->m.p,
+                              (float*)s->v.p, s->count, s->table_count, (float)cfg->lr_table,
+                              (float)cfg->lr_mlp, 0.9f, 0.99f, 1e-15f, ++s->adam_step,
+                              ctx->stream);
+        if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+        ++steps;
+        if (std::isnan(first)) {
+          double l = 0;
+          HIPCHK(hipStreamSynchronize(ctx->stream));
+          HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
+          first = l / (double)nb;
+        }
       }
+      double l = 0;
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
+      epoch_loss = l / (double)n;
     }
-    double l = 0;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipMemcpy(&l, s->loss.p, sizeof l, hipMemcpyDeviceToHost));
-    epoch_loss = l / (double)n;
-  }
-  HIPCHK(hipEventRecord(e1, ctx->stream));
-  HIPCHK(hipEventSynchronize(e1));
-  float train_ms = 0;
-  HIPCHK(hipEventElapsedTime(&train_ms, e0, e1));
-  s->packed = false;
-  if (st) {
-    std::memset(st, 0, sizeof *st);
-    st->first_loss = first;
-    st->final_loss = epoch_loss;
-    st->label_ms = label_ms;
-    st->train_ms = train_ms;
-    st->steps = steps;
-  }
-  return XRP_OK;
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float train_ms = 0;
+    HIPCHK(hipEventElapsedTime(&train_ms, e0, e1));
+    s->packed = false;
+    if (st) {
+      xrp_train_stats o;
+      std::memset(&o, 0, sizeof o);
+      o.first_loss = first;
+      o.final_loss = epoch_loss;
+      o.label_ms = label_ms;
+      o.train_ms = train_ms;
+      o.steps = steps;
+      write_out(st, o);
+    }
+    return XRP_OK;
+  });
 }
 
 int xrp_surrogate_render_device(xrp_surrogate* s, const xrp_render_params* p,
                                 const xrp_view_angle* views, int n_views, double ds_sur,
                                 double tol, void* out_device, xrp_surrogate_stats* st) {
-  using namespace xrp;
-  if (!s || !views || !out_device) return fail(XRP_ERR_NULL, "NULL argument");
-  int rc = check_params(p, n_views);
-  if (rc) return rc;
-  if (p->output_format != XRP_OUTPUT_F32)
-    return fail(XRP_ERR_ARG, "the surrogate writes float32 images");
-  if (!(ds_sur > 0) || !std::isfinite(ds_sur)) return fail(XRP_ERR_ARG, "ds_surrogate > 0");
-  if (n_views > 65535) return fail(XRP_ERR_DIMS, "at most 65535 views per call");
-  xrp_context* ctx = s->ctx;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  RenderJob job = make_job(p, views, n_views);
-  LaunchPlan pl;
-  rc = plan_job(ctx, s->scene, job, pl, /*allow_voxel_kernel=*/false);
-  if (rc) return rc;
-  const Scene& sc = *s->scene->s;
-  const int res = job.res;
-  // exact plan on the context buffers
-  HIPCHK(ctx->table.ensure(std::max<size_t>(1, pl.table.size()) * sizeof(double)));
-  if (!pl.table.empty())
-    HIPCHK(hipMemcpy(ctx->table.p, pl.table.data(), pl.table.size() * sizeof(double),
+  return guarded([&]() -> int {
+    using namespace xrp;
+    if (!s || !views || !out_device) return fail(XRP_ERR_NULL, "NULL argument");
+    int rc = check_params(p, n_views);
+    if (!rc) rc = check_out(st, "stats");
+    if (rc) return rc;
+    if (p->output_format != XRP_OUTPUT_F32)
+      return fail(XRP_ERR_ARG, "the surrogate writes float32 images");
+    if (!(ds_sur > 0) || !std::isfinite(ds_sur)) return fail(XRP_ERR_ARG, "ds_surrogate > 0");
+    if (n_views > 65535) return fail(XRP_ERR_DIMS, "at most 65535 views per call");
+    xrp_context* ctx = s->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    RenderJob job = make_job(p, views, n_views);
+    LaunchPlan pl;
+    rc = plan_job(ctx, s->scene, job, pl, /*allow_voxel_kernel=*/false);
+    if (rc) return rc;
+    const Scene& sc = *s->scene->s;
+    const int res = job.res;
+    // exact plan on the context buffers
+    HIPCHK(ctx->table.ensure(std::max<size_t>(1, pl.table.size()) * sizeof(double)));
+    if (!pl.table.empty())
+      HIPCHK(hipMemcpy(ctx->table.p, pl.table.data(), pl.table.size() * sizeof(double),
+                       hipMemcpyHostToDevice));
+    ctx->table_mode = -1;  // the cached table key no longer matches
+    HIPCHK(ctx->nodes.ensure(pl.nodes.size() * sizeof(DevNode)));
+    HIPCHK(ctx->defs.ensure(std::max<size_t>(1, sc.ddefs.size()) * sizeof(DevDeform)));
+    HIPCHK(ctx->views.ensure(job.views.size() * sizeof(DevView)));
+    HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
+    HIPCHK(hipMemcpy(ctx->nodes.p, pl.nodes.data(), pl.nodes.size() * sizeof(DevNode),
                      hipMemcpyHostToDevice));
-  ctx->table_mode = -1;  // the cached table key no longer matches
-  HIPCHK(ctx->nodes.ensure(pl.nodes.size() * sizeof(DevNode)));
-  HIPCHK(ctx->defs.ensure(std::max<size_t>(1, sc.ddefs.size()) * sizeof(DevDeform)));
-  HIPCHK(ctx->views.ensure(job.views.size() * sizeof(DevView)));
-  HIPCHK(ctx->stats.ensure(ST_COUNT * sizeof(unsigned long long)));
-  HIPCHK(hipMemcpy(ctx->nodes.p, pl.nodes.data(), pl.nodes.size() * sizeof(DevNode),
-                   hipMemcpyHostToDevice));
-  if (!sc.ddefs.empty())
-    HIPCHK(hipMemcpy(ctx->defs.p, sc.ddefs.data(), sc.ddefs.size() * sizeof(DevDeform),
+    if (!sc.ddefs.empty())
+      HIPCHK(hipMemcpy(ctx->defs.p, sc.ddefs.data(), sc.ddefs.size() * sizeof(DevDeform),
+                       hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->views.p, job.views.data(), job.views.size() * sizeof(DevView),
                      hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->views.p, job.views.data(), job.views.size() * sizeof(DevView),
-                   hipMemcpyHostToDevice));
-  MarchParams mp = pl.mp;
-  mp.nodes = (const DevNode*)ctx->nodes.p;
-  mp.defs = (const DevDeform*)ctx->defs.p;
-  mp.views = (const DevView*)ctx->views.p;
-  mp.s_table = (const double*)ctx->table.p;
-  mp.stats = nullptr;
-  mp.out = (float*)out_device;
-  mp.view_offset = 0;
-  mp.n_views = n_views;
-  if (!s->packed) {
-    int e = launch_sur_pack_tables(s->net, ctx->stream);
-    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-    s->packed = true;
-  }
-  xrp_surrogate_stats out_st;
-  std::memset(&out_st, 0, sizeof out_st);
-  HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long), ctx->stream));
-  MarchParams smp = mp;
-  smp.stats = (unsigned long long*)ctx->stats.p;
-  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  int e = launch_sur_render(s->net, smp, ds_sur, ctx->stream);
-  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-  HIPCHK(hipEventSynchronize(ctx->ev1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  out_st.surrogate_ms = ms;
-  unsigned long long nsamp = 0;
-  HIPCHK(hipMemcpy(&nsamp, (unsigned long long*)ctx->stats.p + ST_SAMPLES, sizeof nsamp,
-                   hipMemcpyDeviceToHost));
-  out_st.network_samples = nsamp;
-  const int tiles_side = (res + 15) / 16;
-  out_st.tiles = (uint64_t)tiles_side * tiles_side * n_views;
-  if (tol > 0) {
-    // 4 check pixels per 16x16 tile
-    std::vector<int> items;
-    items.reserve(out_st.tiles * 12);
-    const int offs[4][2] = {{3, 3}, {3, 11}, {11, 3}, {11, 11}};
-    for (int v = 0; v < n_views; ++v)
-      for (int ti = 0; ti < tiles_side; ++ti)
-        for (int tj = 0; tj < tiles_side; ++tj)
-          for (auto& o : offs) {
-            items.push_back(v);
-            items.push_back(std::min(res - 1, ti * 16 + o[0]));
-            items.push_back(std::min(res - 1, tj * 16 + o[1]));
-          }
-    const int n_check = (int)(items.size() / 3);
-    HIPCHK(s->items.ensure(items.size() * sizeof(int)));
-    HIPCHK(s->vals.ensure((size_t)n_check * sizeof(float)));
-    HIPCHK(hipMemcpy(s->items.p, items.data(), items.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
-    const int prec = pl.precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED;
-    const int integ = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
-                                                                   : INT_SIMPLE;
+    MarchParams mp = pl.mp;
+    mp.nodes = (const DevNode*)ctx->nodes.p;
+    mp.defs = (const DevDeform*)ctx->defs.p;
+    mp.views = (const DevView*)ctx->views.p;
+    mp.s_table = (const double*)ctx->table.p;
+    mp.stats = nullptr;
+    mp.out = (float*)out_device;
+    mp.view_offset = 0;
+    mp.n_views = n_views;
+    if (!s->packed) {
+      int e = launch_sur_pack_tables(s->net, ctx->stream);
+      if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+      s->packed = true;
+    }
+    xrp_surrogate_stats out_st;
+    std::memset(&out_st, 0, sizeof out_st);
+    HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long), ctx->stream));
+    MarchParams smp = mp;
+    smp.stats = (unsigned long long*)ctx->stats.p;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    e = launch_march_list(mp, prec, integ, (const int*)s->items.p, n_check,
-                          (float*)s->vals.p, ctx->stream);
+    int e = launch_sur_render(s->net, smp, ds_sur, ctx->stream);
     if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    out_st.check_ms = ms;
-    out_st.check_pixels = (uint64_t)n_check;
-    std::vector<float> exact((size_t)n_check);
-    HIPCHK(hipMemcpy(exact.data(), s->vals.p, exact.size() * sizeof(float),
+    out_st.surrogate_ms = ms;
+    unsigned long long nsamp = 0;
+    HIPCHK(hipMemcpy(&nsamp, (unsigned long long*)ctx->stats.p + ST_SAMPLES, sizeof nsamp,
                      hipMemcpyDeviceToHost));
-    // the surrogate's values at the same pixels (vals is reused)
-    e = launch_gather_pixels((const float*)out_device, (const int*)s->items.p, n_check, res,
-                             (float*)s->vals.p, ctx->stream);
-    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-    std::vector<float> sur((size_t)n_check);
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipMemcpy(sur.data(), s->vals.p, sur.size() * sizeof(float), hipMemcpyDeviceToHost));
-    std::vector<int> fail_tiles;
-    double worst = 0;
-    for (int t = 0; t < n_check / 4; ++t) {
-      double err = 0;
-      for (int q = 0; q < 4; ++q)
-        err = std::max(err, (double)std::fabs(sur[4 * t + q] - exact[4 * t + q]));
-      worst = std::max(worst, err);
-      if (err > tol) {
-        fail_tiles.push_back(items[12 * t]);
-        fail_tiles.push_back(items[12 * t + 1] / 16);
-        fail_tiles.push_back(items[12 * t + 2] / 16);
-      }
-    }
-    out_st.max_sampled_error = worst;
-    out_st.tiles_failed = fail_tiles.size() / 3;
-    if (!fail_tiles.empty()) {
-      HIPCHK(s->items.ensure(fail_tiles.size() * sizeof(int)));
-      HIPCHK(hipMemcpy(s->items.p, fail_tiles.data(), fail_tiles.size() * sizeof(int),
+    out_st.network_samples = nsamp;
+    const int tiles_side = (res + 15) / 16;
+    out_st.tiles = (uint64_t)tiles_side * tiles_side * n_views;
+    if (tol > 0) {
+      // kCheck check pixels per 16x16 tile on a stride-4 grid (rows and
+      // columns 1, 5, 9, 13 of the tile); a tile passes when every one is
+      // within tol / 2, the other half of the tolerance being the margin
+      // left for the pixels between them
+      constexpr int kCheck = 16;
+      std::vector<int> items;
+      items.reserve(out_st.tiles * 3 * kCheck);
+      for (int v = 0; v < n_views; ++v)
+        for (int ti = 0; ti < tiles_side; ++ti)
+          for (int tj = 0; tj < tiles_side; ++tj)
+            for (int c = 0; c < kCheck; ++c) {
+              items.push_back(v);
+              items.push_back(std::min(res - 1, ti * 16 + 1 + 4 * (c >> 2)));
+              items.push_back(std::min(res - 1, tj * 16 + 1 + 4 * (c & 3)));
+            }
+      const int n_check = (int)(items.size() / 3);
+      HIPCHK(s->items.ensure(items.size() * sizeof(int)));
+      HIPCHK(s->vals.ensure((size_t)n_check * sizeof(float)));
+      HIPCHK(hipMemcpy(s->items.p, items.data(), items.size() * sizeof(int),
                        hipMemcpyHostToDevice));
+      // the exact marches count their samples (evaluated + settled) here
+      HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long),
+                            ctx->stream));
+      mp.stats = (unsigned long long*)ctx->stats.p;
+      const int prec = pl.precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED;
+      const int integ = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
+                                                                     : INT_SIMPLE;
       HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-      e = launch_march_list(mp, prec, integ, (const int*)s->items.p,
-                            (int)(fail_tiles.size() / 3), nullptr, ctx->stream);
+      e = launch_march_list(mp, prec, integ, (const int*)s->items.p, n_check,
+                            (float*)s->vals.p, ctx->stream);
       if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
       HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
       HIPCHK(hipEventSynchronize(ctx->ev1));
       HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-      out_st.fallback_ms = ms;
+      out_st.check_ms = ms;
+      out_st.check_pixels = (uint64_t)n_check;
+      std::vector<float> exact((size_t)n_check);
+      HIPCHK(hipMemcpy(exact.data(), s->vals.p, exact.size() * sizeof(float),
+                       hipMemcpyDeviceToHost));
+      // the surrogate's values at the same pixels (vals is reused)
+      e = launch_gather_pixels((const float*)out_device, (const int*)s->items.p, n_check, res,
+                               (float*)s->vals.p, ctx->stream);
+      if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+      std::vector<float> sur((size_t)n_check);
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      HIPCHK(hipMemcpy(sur.data(), s->vals.p, sur.size() * sizeof(float), hipMemcpyDeviceToHost));
+      std::vector<int> fail_tiles;
+      double worst = 0;
+      for (int t = 0; t < n_check / kCheck; ++t) {
+        double err = 0;
+        for (int q = 0; q < kCheck; ++q)
+          err = std::max(err, (double)std::fabs(sur[kCheck * t + q] - exact[kCheck * t + q]));
+        worst = std::max(worst, err);
+        if (err > 0.5 * tol) {
+          fail_tiles.push_back(items[3 * kCheck * t]);
+          fail_tiles.push_back(items[3 * kCheck * t + 1] / 16);
+          fail_tiles.push_back(items[3 * kCheck * t + 2] / 16);
+        }
+      }
+      out_st.max_sampled_error = worst;
+      out_st.tiles_failed = fail_tiles.size() / 3;
+      if (!fail_tiles.empty()) {
+        HIPCHK(s->items.ensure(fail_tiles.size() * sizeof(int)));
+        HIPCHK(hipMemcpy(s->items.p, fail_tiles.data(), fail_tiles.size() * sizeof(int),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        e = launch_march_list(mp, prec, integ, (const int*)s->items.p,
+                              (int)(fail_tiles.size() / 3), nullptr, ctx->stream);
+        if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(hipEventSynchronize(ctx->ev1));
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        out_st.fallback_ms = ms;
+      }
+      unsigned long long cnt[ST_COUNT];
+      HIPCHK(hipMemcpy(cnt, ctx->stats.p, sizeof cnt, hipMemcpyDeviceToHost));
+      out_st.exact_samples = cnt[ST_SAMPLES] - cnt[ST_SKIPPED];
     }
-  }
-  if (st) *st = out_st;
-  return XRP_OK;
+    write_out(st, out_st);
+    return XRP_OK;
+  });
 }
 
 int xrp_surrogate_render(xrp_surrogate* s, const xrp_render_params* p,
                          const xrp_view_angle* views, int n_views, double ds_sur, double tol,
                          float* out_host, xrp_surrogate_stats* st) {
-  if (!s || !p || !out_host) return fail(XRP_ERR_NULL, "NULL argument");
-  int rc = check_params(p, n_views);
-  if (rc) return rc;
-  const size_t bytes = (size_t)n_views * p->resolution * p->resolution * sizeof(float);
-  void* dev = nullptr;
-  {
+  return guarded([&]() -> int {
+    if (!s || !p || !out_host) return fail(XRP_ERR_NULL, "NULL argument");
+    int rc = check_params(p, n_views);
+    if (rc) return rc;
+    const size_t bytes = (size_t)n_views * p->resolution * p->resolution * sizeof(float);
+    void* dev = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(s->ctx->mu);
+      HIPCHK(hipSetDevice(s->ctx->device));
+      HIPCHK(s->ctx->out.ensure(bytes));
+      dev = s->ctx->out.p;
+    }
+    rc = xrp_surrogate_render_device(s, p, views, n_views, ds_sur, tol, dev, st);
+    if (rc) return rc;
     std::lock_guard<std::mutex> lk(s->ctx->mu);
-    HIPCHK(hipSetDevice(s->ctx->device));
-    HIPCHK(s->ctx->out.ensure(bytes));
-    dev = s->ctx->out.p;
-  }
-  rc = xrp_surrogate_render_device(s, p, views, n_views, ds_sur, tol, dev, st);
-  if (rc) return rc;
-  std::lock_guard<std::mutex> lk(s->ctx->mu);
-  HIPCHK(hipMemcpy(out_host, dev, bytes, hipMemcpyDeviceToHost));
-  return XRP_OK;
+    HIPCHK(hipMemcpy(out_host, dev, bytes, hipMemcpyDeviceToHost));
+    return XRP_OK;
+  });
 }
 
 void xrp_surrogate_destroy(xrp_surrogate* s) {
@@ -1910,81 +2067,87 @@ void xrp_surrogate_destroy(xrp_surrogate* s) {
 }
 
 int xrp_diag_mfma_bf16(xrp_context* ctx, const float* a, const float* b, float* d) {
-  if (!ctx || !a || !b || !d) return fail(XRP_ERR_NULL, "NULL argument");
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  DevBuf buf;
-  HIPCHK(buf.ensure(3 * 256 * sizeof(float)));
-  float* p = (float*)buf.p;
-  hipError_t he = hipMemcpy(p, a, 256 * sizeof(float), hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMemcpy(p + 256, b, 256 * sizeof(float), hipMemcpyHostToDevice);
-  int e = he == hipSuccess ? xrp::launch_mfma_probe(p, p + 256, p + 512, ctx->stream) : 0;
-  if (he == hipSuccess && e == 0) he = hipStreamSynchronize(ctx->stream);
-  if (he == hipSuccess && e == 0)
-    he = hipMemcpy(d, p + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
-  buf.release();
-  if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
-  if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !a || !b || !d) return fail(XRP_ERR_NULL, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    HIPCHK(buf.ensure(3 * 256 * sizeof(float)));
+    float* p = (float*)buf.p;
+    hipError_t he = hipMemcpy(p, a, 256 * sizeof(float), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(p + 256, b, 256 * sizeof(float), hipMemcpyHostToDevice);
+    int e = he == hipSuccess ? xrp::launch_mfma_probe(p, p + 256, p + 512, ctx->stream) : 0;
+    if (he == hipSuccess && e == 0) he = hipStreamSynchronize(ctx->stream);
+    if (he == hipSuccess && e == 0)
+      he = hipMemcpy(d, p + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
+    buf.release();
+    if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+    if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+    return XRP_OK;
+  });
 }
 
 int xrp_diag_sincos_error(xrp_context* ctx, double* max_sin, double* max_cos) {
-  if (!ctx || !max_sin || !max_cos) return fail(XRP_ERR_NULL, "NULL argument");
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  DevBuf out;
-  HIPCHK(out.ensure(2 * sizeof(unsigned)));
-  hipError_t he = hipMemsetAsync(out.p, 0, 2 * sizeof(unsigned), ctx->stream);
-  if (he == hipSuccess) {
-    int e = xrp::launch_sincos_error((unsigned*)out.p, ctx->stream);
-    if (e) he = (hipError_t)e;
-  }
-  unsigned h[2] = {0, 0};
-  if (he == hipSuccess)
-    he = hipMemcpyAsync(h, out.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
-  if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-  out.release();
-  if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
-  float fs, fc;
-  std::memcpy(&fs, &h[0], sizeof fs);
-  std::memcpy(&fc, &h[1], sizeof fc);
-  *max_sin = fs;
-  *max_cos = fc;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !max_sin || !max_cos) return fail(XRP_ERR_NULL, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf out;
+    HIPCHK(out.ensure(2 * sizeof(unsigned)));
+    hipError_t he = hipMemsetAsync(out.p, 0, 2 * sizeof(unsigned), ctx->stream);
+    if (he == hipSuccess) {
+      int e = xrp::launch_sincos_error((unsigned*)out.p, ctx->stream);
+      if (e) he = (hipError_t)e;
+    }
+    unsigned h[2] = {0, 0};
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(h, out.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    out.release();
+    if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+    float fs, fc;
+    std::memcpy(&fs, &h[0], sizeof fs);
+    std::memcpy(&fc, &h[1], sizeof fc);
+    *max_sin = fs;
+    *max_cos = fc;
+    return XRP_OK;
+  });
 }
 
 int xrp_diag_l2_bandwidth(xrp_context* ctx, double* gbps) {
-  if (!ctx || !gbps) return fail(XRP_ERR_NULL, "NULL argument");
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  hipDeviceProp_t prop;
-  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
-  const int workgroups = 8 * std::max(1, prop.multiProcessorCount);
-  const int iters = 4096;
-  const size_t bytes = xrp::l2_probe_buffer_bytes();
-  DevBuf buf, out;
-  HIPCHK(buf.ensure(bytes));
-  HIPCHK(out.ensure((size_t)workgroups * 256 * sizeof(unsigned)));
-  hipError_t he = hipMemsetAsync(buf.p, 1, bytes, ctx->stream);
-  float best = 0.0f;
-  for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {
-    he = hipEventRecord(ctx->ev0, ctx->stream);
-    int e = he == hipSuccess ? xrp::launch_l2_probe(buf.p, workgroups, iters,
-                                                    (unsigned*)out.p, ctx->stream)
-                             : 0;
-    if (e) he = (hipError_t)e;
-    if (he == hipSuccess) he = hipEventRecord(ctx->ev1, ctx->stream);
-    if (he == hipSuccess) he = hipEventSynchronize(ctx->ev1);
-    float ms = 0.0f;
-    if (he == hipSuccess) he = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    if (he == hipSuccess && rep > 0 && ms > 0.0f && (best == 0.0f || ms < best)) best = ms;
-  }
-  buf.release();
-  out.release();
-  if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
-  const double read = (double)workgroups * 256.0 * iters * 16.0;
-  *gbps = best > 0.0f ? read / (best * 1e-3) / 1e9 : 0.0;
-  return XRP_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !gbps) return fail(XRP_ERR_NULL, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+    const int workgroups = 8 * std::max(1, prop.multiProcessorCount);
+    const int iters = 4096;
+    const size_t bytes = xrp::l2_probe_buffer_bytes();
+    DevBuf buf, out;
+    HIPCHK(buf.ensure(bytes));
+    HIPCHK(out.ensure((size_t)workgroups * 256 * sizeof(unsigned)));
+    hipError_t he = hipMemsetAsync(buf.p, 1, bytes, ctx->stream);
+    float best = 0.0f;
+    for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {
+      he = hipEventRecord(ctx->ev0, ctx->stream);
+      int e = he == hipSuccess ? xrp::launch_l2_probe(buf.p, workgroups, iters,
+                                                      (unsigned*)out.p, ctx->stream)
+                               : 0;
+      if (e) he = (hipError_t)e;
+      if (he == hipSuccess) he = hipEventRecord(ctx->ev1, ctx->stream);
+      if (he == hipSuccess) he = hipEventSynchronize(ctx->ev1);
+      float ms = 0.0f;
+      if (he == hipSuccess) he = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+      if (he == hipSuccess && rep > 0 && ms > 0.0f && (best == 0.0f || ms < best)) best = ms;
+    }
+    buf.release();
+    out.release();
+    if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+    const double read = (double)workgroups * 256.0 * iters * 16.0;
+    *gbps = best > 0.0f ? read / (best * 1e-3) / 1e9 : 0.0;
+    return XRP_OK;
+  });
 }
 
 }  // extern "C"

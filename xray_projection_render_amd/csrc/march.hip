@@ -137,11 +137,14 @@ int launch_march_list(const MarchParams& p, int precision, int integrator, const
 __global__ __launch_bounds__(256) void volume_density_kernel(MarchParams p, int res,
                                                              double* out,
                                                              unsigned long long* maxbits) {
-  const size_t n = (size_t)res * res * res;
-  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  // blockIdx.y is the slice k; blockIdx.x covers its res*res voxels
+  const unsigned slice = (unsigned)res * (unsigned)res;
+  const unsigned ij = blockIdx.x * 256u + threadIdx.x;
+  const size_t k = blockIdx.y;
   double v = 0.0;
-  if (idx < n) {
-    size_t k = idx / ((size_t)res * res), i = (idx / res) % res, j = idx % res;
+  if (ij < slice) {
+    const size_t idx = k * slice + ij;
+    const unsigned i = ij / (unsigned)res, j = ij % (unsigned)res;
     double x = (double)i / (double)res * 2.0 - 1.0;
     double y = (double)j / (double)res * 2.0 - 1.0;
     double z = (double)k / (double)res * 2.0 - 1.0;
@@ -161,11 +164,13 @@ __global__ __launch_bounds__(256) void volume_density_kernel(MarchParams p, int 
 
 // byte(v / max_val * 255): Go converts through int64 (CVTTSD2SQ on amd64)
 // and keeps the low byte; NaN and out-of-range values give 0.
-__global__ __launch_bounds__(256) void volume_quantize_kernel(const double* dens, size_t n,
+__global__ __launch_bounds__(256) void volume_quantize_kernel(const double* dens,
+                                                              unsigned slice,
                                                               const unsigned long long* maxbits,
                                                               unsigned char* out) {
-  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n) return;
+  const unsigned ij = blockIdx.x * 256u + threadIdx.x;
+  if (ij >= slice) return;
+  const size_t idx = (size_t)blockIdx.y * slice + ij;
   double mx = __longlong_as_double((long long)*maxbits);
   double q = dens[idx] / mx * 255;
   long long iq = (q == q && q > -9.2e18 && q < 9.2e18) ? (long long)q : 0;
@@ -203,11 +208,12 @@ int launch_sincos_error(unsigned* out, void* stream) {
 int launch_volume(const MarchParams& p, int res, double* dens, unsigned long long* maxbits,
                   unsigned char* out, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  size_t n = (size_t)res * res * res;
-  unsigned blocks = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(volume_density_kernel, dim3(blocks), dim3(256), 0, s, p, res, dens,
-                     maxbits);
-  hipLaunchKernelGGL(volume_quantize_kernel, dim3(blocks), dim3(256), 0, s, dens, n, maxbits,
+  // one grid row per z slice: res <= 2048 keeps both grid dimensions in
+  // range (x: res^2/256 <= 16384 blocks, y: res <= 65535)
+  const unsigned slice = (unsigned)res * (unsigned)res;
+  const dim3 grid((slice + 255) / 256, (unsigned)res);
+  hipLaunchKernelGGL(volume_density_kernel, grid, dim3(256), 0, s, p, res, dens, maxbits);
+  hipLaunchKernelGGL(volume_quantize_kernel, grid, dim3(256), 0, s, dens, slice, maxbits,
                      out);
   return (int)hipGetLastError();
 }
