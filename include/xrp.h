@@ -192,6 +192,23 @@ int xrp_render_device(xrp_context *ctx, xrp_scene *scene,
                       const xrp_render_params *p, const xrp_view_angle *views,
                       int n_views, void *out_device, xrp_render_stats *stats);
 
+/* No counterpart in the reference.  xrp_render_device for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): this call allocates no device or pinned host
+ * memory.  Its per-call data (stat counters, node table, deformations,
+ * views and sample table) goes into a fixed 2 MiB region of device memory,
+ * one per GPU, held until the process ends; larger requests (for example
+ * more than about 12,000 views in one call, or a sample table above
+ * 250,000 entries) fail with XRP_ERR_DIMS.  A scene holding a voxel grid
+ * must first be rendered once on the same context with xrp_render or
+ * xrp_render_device (which upload the grid), else XRP_ERR_UNSUPPORTED.
+ * Every call on one GPU shares that GPU's region: calls that overlap on one
+ * GPU (from different contexts) race; calls on one context are serialized.
+ * The output is the same as xrp_render_device's. */
+int xrp_render_device_noalloc(xrp_context *ctx, xrp_scene *scene,
+                              const xrp_render_params *p, const xrp_view_angle *views,
+                              int n_views, void *out_device, xrp_render_stats *stats);
+
 /* Explicit-camera voxel render replacing RenderVolumeProjectionsCUDA
  * (cuda_backend.h:70-112) with the CPU renderer's semantics: software
  * trilinear with the (N-1) convention, FP64 accumulated sample positions.

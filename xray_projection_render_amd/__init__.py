@@ -9,7 +9,7 @@ keeps the reference's Python interface.
 from ._lib import (Batch, Context, Scene, Surrogate, XrpError, camera_from_angles,
                    diag_l2_bandwidth, diag_mfma_bf16, diag_sincos_error, export_volume,
                    join_surrogate_params,
-                   lib, render, render_device, render_volume_projections,
+                   lib, render, render_device, render_device_noalloc, render_volume_projections,
                    sample_count,
                    split_surrogate_params)
 from .xray_renderer import XRayRenderer
@@ -17,5 +17,5 @@ from .xray_renderer import XRayRenderer
 __all__ = ["Batch", "Context", "Scene", "Surrogate", "XRayRenderer", "XrpError",
            "camera_from_angles", "diag_l2_bandwidth", "diag_mfma_bf16", "diag_sincos_error",
            "export_volume",
-           "join_surrogate_params", "lib", "render", "render_device",
+           "join_surrogate_params", "lib", "render", "render_device", "render_device_noalloc",
            "render_volume_projections", "sample_count", "split_surrogate_params"]

@@ -179,6 +179,9 @@ def lib() -> ctypes.CDLL:
         "xrp_render_device": (I, [P, P, ctypes.POINTER(RenderParams),
                                   ctypes.POINTER(ViewAngle), I, P,
                                   ctypes.POINTER(RenderStats)]),
+        "xrp_render_device_noalloc": (I, [P, P, ctypes.POINTER(RenderParams),
+                                          ctypes.POINTER(ViewAngle), I, P,
+                                          ctypes.POINTER(RenderStats)]),
         "xrp_render_volume_projections": (I, [P, FP, I, I, I, ctypes.POINTER(Camera), I, I,
                                               D, D, FP]),
         "xrp_export_volume": (I, [P, P, I, D, ctypes.POINTER(ctypes.c_uint8)]),
@@ -405,6 +408,18 @@ def render_device(ctx: Context, scene: Scene, angles: Sequence[Tuple[float, floa
     st = RenderStats()
     check(lib().xrp_render_device(ctx.ptr, scene.ptr, ctypes.byref(p), _views(angles),
                                   len(angles), ctypes.c_void_p(out_ptr), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def render_device_noalloc(ctx: Context, scene: Scene, angles: Sequence[Tuple[float, float]],
+                          resolution: int, ds: float, out_ptr: int, **kw) -> Dict:
+    """render_device without any allocation in the library (its per-call data
+    goes into a fixed per-GPU scratch region; xrp_render_device_noalloc)."""
+    p = make_params(resolution, ds, **kw)
+    st = RenderStats()
+    check(lib().xrp_render_device_noalloc(ctx.ptr, scene.ptr, ctypes.byref(p), _views(angles),
+                                          len(angles), ctypes.c_void_p(out_ptr),
+                                          ctypes.byref(st)))
     return st.as_dict()
 
 

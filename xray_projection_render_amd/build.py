@@ -34,6 +34,7 @@ SOURCES = [
     ("voxel.hip", True),
     ("batch.hip", True),
     ("surrogate.hip", True),
+    ("noalloc.hip", True),
     ("api.cpp", False),
     ("scene.cpp", False),
     ("camera.cpp", False),

@@ -135,6 +135,8 @@ struct xrp_context {
   // xrp_render_volume_projections: the last volume as a scene, reused while
   // the caller passes the same grid
   xrp_scene* vol_scene = nullptr;
+  // the fixed scratch region of xrp_render_device_noalloc on this device
+  void* scratch = nullptr;
 };
 
 struct xrp_scene {
@@ -202,6 +204,7 @@ int xrp_context_create(int device, xrp_context** out) {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = (hipError_t)xrp::noalloc_scratch(device, &c->scratch);
     if (e != hipSuccess) {
       delete c;
       return fail(XRP_ERR_DEVICE, std::string("context set-up failed: ") +
@@ -514,8 +517,11 @@ enum BrickSlot { SLOT_PLAIN = 0, SLOT_PAIR = 1, SLOT_QUAD = 2, N_SLOTS = 3 };
 // use (slot < 0 asks for none), so a scene holds only the copies its
 // layouts need.
 int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs,
-                  int slot = -1) {
+                  int slot = -1, bool no_alloc = false) {
   std::lock_guard<std::mutex> g(g_registry_mu);
+  const char* not_resident =
+      "voxel grid not resident on this context: render the scene once with "
+      "xrp_render or xrp_render_device first (this call allocates nothing)";
   const bool root_vox = sc->s->root.kind == xrp::K_VOXEL;
   auto put = [](const std::vector<float>& host, size_t bytes, void*& p) -> hipError_t {
     p = nullptr;
@@ -528,6 +534,7 @@ int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs,
     return e;
   };
   auto it = sc->uploads.find(ctx);
+  if (it == sc->uploads.end() && no_alloc) return fail(XRP_ERR_UNSUPPORTED, not_resident);
   if (it == sc->uploads.end()) {
     std::vector<const xrp::HNode*> vox;
     std::function<void(const xrp::HNode&)> walk = [&](const xrp::HNode& n) {
@@ -553,6 +560,7 @@ int upload_voxels(xrp_context* ctx, xrp_scene* sc, std::vector<void*>& ptrs,
   std::vector<void*>& out = it->second;
   if (root_vox && slot >= 0) {
     void*& p = out[out.size() - N_SLOTS + slot];
+    if (!p && no_alloc) return fail(XRP_ERR_UNSUPPORTED, not_resident);
     if (!p) {
       const xrp::HNode& n = sc->s->root;
       std::vector<float> br;
@@ -591,7 +599,7 @@ struct LaunchPlan {
 };
 
 int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& pl,
-             bool allow_voxel_kernel = true) {
+             bool allow_voxel_kernel = true, bool no_alloc = false) {
   using namespace xrp;
   Scene& s = *sc->s;
   const int res = job.res;
@@ -655,7 +663,7 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
 
   std::vector<void*> vptr;
   if (s.has_voxels) {
-    int rc = upload_voxels(ctx, sc, vptr);
+    int rc = upload_voxels(ctx, sc, vptr, -1, no_alloc);
     if (rc) return rc;
   }
   pl.nodes = s.nodes;
@@ -745,7 +753,7 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
     const int width = layout == VOX_BRICK4_PAIR ? 2 : layout >= VOX_BRICK4_QUAD ? 4 : 1;
     const int slot = width == 1 ? SLOT_PLAIN : width == 2 ? SLOT_PAIR : SLOT_QUAD;
     if (!lin) {
-      int rc = upload_voxels(ctx, sc, vptr, slot);
+      int rc = upload_voxels(ctx, sc, vptr, slot, no_alloc);
       if (rc) return rc;
     }
     const size_t nb = vptr.size();
@@ -777,15 +785,20 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   return XRP_OK;
 }
 
+// With `fixed` (the device address of a region of kNoAllocScratchBytes),
+// the render allocates nothing: stat counters, node table, deformations,
+// views and the sample table are staged from pageable host memory into that
+// region instead of the context's buffers, and a voxel grid must already be
+// resident on the context.
 int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
-                void* out_dev, xrp_render_stats* st) {
+                void* out_dev, xrp_render_stats* st, void* fixed = nullptr) {
   using namespace xrp;
   Scene& s = *sc->s;
   const int nv = (int)job.views.size();
   const int res = job.res;
   HIPCHK(hipSetDevice(ctx->device));
   LaunchPlan pl;
-  int rc = plan_job(ctx, sc, job, pl);
+  int rc = plan_job(ctx, sc, job, pl, true, fixed != nullptr);
   if (rc) return rc;
   const int precision = pl.precision;
   const bool vox_kernel = pl.vox_kernel;
@@ -794,8 +807,8 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
 
   // Sample table (device copy cached per context).
   const double smin = job.R - kCubeHalfDiagonal, smax = job.R + kCubeHalfDiagonal;
-  if (ctx->table_mode != job.integrator || ctx->table_smin != smin ||
-      ctx->table_smax != smax || ctx->table_ds != job.ds) {
+  if (!fixed && (ctx->table_mode != job.integrator || ctx->table_smin != smin ||
+                 ctx->table_smax != smax || ctx->table_ds != job.ds)) {
     ctx->table_host = pl.table;
     HIPCHK(ctx->table.ensure(std::max<size_t>(1, ctx->table_host.size()) *
                              sizeof(double)));
@@ -818,22 +831,43 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   const size_t b_views = (size_t)nv * sizeof(DevView);
   const size_t b_stats = ST_COUNT * sizeof(unsigned long long);
   const size_t o_defs = align(b_nodes), o_views = o_defs + align(std::max<size_t>(b_defs, 1));
-  const size_t o_stats = o_views + align(b_views), total = o_stats + b_stats;
-  HIPCHK(ctx->staging.ensure(total));
-  HIPCHK(ctx->upload.ensure(total));
-  unsigned char* hb = (unsigned char*)ctx->staging.p;
+  const size_t o_stats = o_views + align(b_views);
+  // the no-allocation form also carries the sample table
+  const size_t o_table = o_stats + align(b_stats);
+  const size_t total = fixed ? o_table + pl.table.size() * sizeof(double) : o_stats + b_stats;
+  std::vector<unsigned char> pageable;
+  unsigned char *hb, *db;
+  if (fixed) {
+    if (total > kNoAllocScratchBytes)
+      return fail(XRP_ERR_DIMS,
+                  "nodes, views and sample table need " + std::to_string(total) +
+                      " bytes; the no-allocation scratch region holds " +
+                      std::to_string(kNoAllocScratchBytes));
+    pageable.assign(total, 0);
+    hb = pageable.data();
+    db = (unsigned char*)fixed;
+    if (!pl.table.empty())
+      std::memcpy(hb + o_table, pl.table.data(), pl.table.size() * sizeof(double));
+  } else {
+    HIPCHK(ctx->staging.ensure(total));
+    HIPCHK(ctx->upload.ensure(total));
+    hb = (unsigned char*)ctx->staging.p;
+    db = (unsigned char*)ctx->upload.p;
+  }
   std::memcpy(hb, pl.nodes.data(), b_nodes);
   if (b_defs) std::memcpy(hb + o_defs, s.ddefs.data(), b_defs);
   std::memcpy(hb + o_views, job.views.data(), b_views);
   std::memset(hb + o_stats, 0, b_stats);
-  HIPCHK(hipMemcpyAsync(ctx->upload.p, hb, total, hipMemcpyHostToDevice, ctx->stream));
-  unsigned char* db = (unsigned char*)ctx->upload.p;
+  HIPCHK(hipMemcpyAsync(db, hb, total, hipMemcpyHostToDevice, ctx->stream));
+  // pageable staging: wait for the copy, so that no early return below can
+  // free the vector while the copy still reads it
+  if (fixed) HIPCHK(hipStreamSynchronize(ctx->stream));
 
   MarchParams mp = pl.mp;
   mp.nodes = (const DevNode*)db;
   mp.defs = (const DevDeform*)(db + o_defs);
   mp.views = (const DevView*)(db + o_views);
-  mp.s_table = (const double*)ctx->table.p;
+  mp.s_table = fixed ? (const double*)(db + o_table) : (const double*)ctx->table.p;
   mp.stats = (unsigned long long*)(db + o_stats);
   (void)res;
 
@@ -958,6 +992,21 @@ int xrp_render_device(xrp_context* ctx, xrp_scene* sc,
     std::lock_guard<std::mutex> g(ctx->mu);
     RenderJob job = make_job(p, views, n_views);
     return render_core(ctx, sc, job, out_device, st);
+  });
+}
+
+int xrp_render_device_noalloc(xrp_context* ctx, xrp_scene* sc, const xrp_render_params* p,
+                              const xrp_view_angle* views, int n_views, void* out_device,
+                              xrp_render_stats* st) {
+  return guarded([&]() -> int {
+    if (!ctx || !sc || !views || !out_device) return fail(XRP_ERR_NULL, "NULL argument");
+    int rc = check_params(p, n_views);
+    if (!rc) rc = check_out(st, "stats");
+    if (rc) return rc;
+    if (!ctx->scratch) return fail(XRP_ERR_DEVICE, "no scratch region on this device");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    RenderJob job = make_job(p, views, n_views);
+    return render_core(ctx, sc, job, out_device, st, ctx->scratch);
   });
 }
 

@@ -152,4 +152,10 @@ int launch_volume(const MarchParams& p, int res, double* dens,
 int launch_march(const MarchParams& p, int precision, int integrator,
                  void* stream);
 
+// Fixed per-GPU scratch region of xrp_render_device_noalloc (noalloc.hip):
+// its size, and its device address on `device`, which must be current (the
+// address is looked up once per device and kept).  Returns hipError_t.
+constexpr size_t kNoAllocScratchBytes = (size_t)2 << 20;
+int noalloc_scratch(int device, void** out);
+
 }  // namespace xrp
