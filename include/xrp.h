@@ -356,11 +356,17 @@ int xrp_surrogate_create(xrp_context *ctx, const xrp_scene *scene,
 int xrp_surrogate_train(xrp_surrogate *s, const xrp_train_config *config,
                         xrp_train_stats *stats);
 /* Renders with the network at step ds_surrogate into device memory (float32
- * transmittance [view][i][j]); with tile_tolerance > 0 every 16x16 tile is
- * checked at 16 pixels (a stride-4 grid) against the exact march
- * (params->ds, integrator, precision) and re-rendered exactly when any
- * differs by more than tile_tolerance / 2.  The check is a sample: pixels
- * between check pixels of a tile that passed are not compared. */
+ * transmittance [view][i][j]).  The exact march (params->ds, integrator,
+ * precision) judges the result:
+ *  tile_tolerance > 0  sampled: every 16x16 tile is checked at 16 pixels (a
+ *                      stride-4 grid) and re-rendered exactly when any
+ *                      differs by more than tile_tolerance / 2; pixels
+ *                      between check pixels are not compared;
+ *  tile_tolerance < 0  certified: the exact march renders every pixel and
+ *                      every tile whose largest difference exceeds
+ *                      |tile_tolerance| takes the exact values, so no pixel
+ *                      is further than |tile_tolerance| from the exact render;
+ *  tile_tolerance = 0  the network's image as it is. */
 int xrp_surrogate_render_device(xrp_surrogate *s, const xrp_render_params *params,
                                 const xrp_view_angle *views, int n_views,
                                 double ds_surrogate, double tile_tolerance,

@@ -290,7 +290,7 @@ def blob256():
 
 
 VOXEL_LAYOUTS = ["lds", "brick4", "linear", "cached", "linear_cached", "pair", "quad",
-                 "quad_slide"]
+                 "quad_slide", "quad_step"]
 
 
 @pytest.mark.parametrize("layout", VOXEL_LAYOUTS)
@@ -336,7 +336,7 @@ def test_voxel_uneven_grid_all_layouts(ctx, monkeypatch):
     angles = [(17.0, 63.0), (100.0, 90.0), (250.0, 131.0), (333.0, 45.0)]
     imgs = {}
     for layout in ["brick4", "linear", "cached", "linear_cached", "pair", "quad", "quad_slide",
-                   "lds"]:
+                   "lds", "quad_step"]:
         monkeypatch.setenv("XRP_VOXEL_LAYOUT", layout)
         err, st, img, ref = _compare(ctx, None, angles, 48, 0.0123, deformation=deformation,
                                      voxels=grid, audit=False)
@@ -344,11 +344,34 @@ def test_voxel_uneven_grid_all_layouts(ctx, monkeypatch):
         assert ref.min() < 0.9  # the grid absorbs
         imgs[layout] = img
     for layout, img in imgs.items():
-        if layout != "lds":
+        # lds and quad_step round their positions differently (same error size)
+        if layout not in ("lds", "quad_step"):
             assert np.array_equal(img, imgs["brick4"]), layout
 
 
-@pytest.mark.parametrize("layout", ["brick4", "cached", "pair", "quad", "quad_slide"])
+def test_voxel_quad_step_multi_cell_steps(ctx, monkeypatch):
+    """Steps longer than a cell (2.5 to 7 cells per sample on a 64^3 grid):
+    the stepping march moves the cell by the weights' floors, several cells
+    at once, and still matches the oracle and the gathering layouts."""
+    rng = np.random.default_rng(11)
+    n = 64
+    z, x, y = np.meshgrid(*(np.linspace(-1, 1, n),) * 3, indexing="ij")
+    grid = (0.5 + 0.45 * np.sin(4.1 * x - 2.2 * y + 1.3 * z)
+            + 0.05 * rng.random((n, n, n))).astype(np.float32)
+    angles = [(31.0, 80.0), (200.0, 110.0)]
+    for ds in (0.08, 0.2):
+        out = {}
+        for layout in ("quad_step", "quad_slide"):
+            monkeypatch.setenv("XRP_VOXEL_LAYOUT", layout)
+            err, _, img, _ = _compare(ctx, None, angles, 40, ds,
+                                      deformation=W.VOXEL_DEFORMATION, voxels=grid, audit=False)
+            assert err <= TOL, (layout, ds)
+            out[layout] = img
+        assert float(np.abs(out["quad_step"] - out["quad_slide"]).max()) <= TOL
+
+
+@pytest.mark.parametrize("layout", ["brick4", "cached", "pair", "quad", "quad_slide",
+                                    "quad_step"])
 def test_voxel_boundary_ties(ctx, layout, monkeypatch):
     """Axis-aligned views put samples on the grid faces x = +-1 exactly;
     the per-ray FP64 range must count them as the reference does."""

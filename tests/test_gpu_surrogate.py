@@ -147,31 +147,54 @@ def c5_reduced(ctx):
     sc.close()
 
 
-def test_surrogate_with_fallback_within_tolerance_everywhere(c5_reduced):
-    """After the tile check and exact fallback, every pixel of the image is
-    within 1e-3 of the exact render (not only the check pixels)."""
+def test_certified_surrogate_within_tolerance_everywhere(c5_reduced):
+    """Certified check (negative tolerance): every pixel of the result is
+    within 1e-3 of the exact render, whatever the network does; the tiles
+    the network gets right keep its values."""
+    w, sc, exact, _, sur, _ = c5_reduced
+    tol = 1e-3
+    img, rs = sur.render(w.angles, 128, w.ds, 2 * w.ds, -tol, integrator="simple")
+    err = np.abs(img.astype(np.float64) - exact)
+    print(f"certified: full image max {err.max():.3e}; tiles {rs['tiles']} replaced "
+          f"{rs['tiles_failed']}; {rs}", flush=True)
+    assert float(err.max()) <= tol
+    assert rs["check_pixels"] == 4 * 128 * 128
+    pure, _ = sur.render(w.angles, 128, w.ds, 2 * w.ds, 0.0, integrator="simple")
+    perr = np.abs(pure.astype(np.float64) - exact)
+    # the largest difference the check saw is the network's own
+    assert abs(rs["max_sampled_error"] - float(perr.max())) <= 1e-6
+    kept = rs["tiles"] - rs["tiles_failed"]
+    print(f"pure surrogate max {perr.max():.3e}, pixels above tol {(perr > tol).mean():.3%}; "
+          f"{kept} tiles keep the network's values", flush=True)
+    assert kept > 0
+
+
+def test_sampled_check_escapes_are_measured(c5_reduced):
+    """Sampled check (positive tolerance): 16 pixels per tile.  Check pixels
+    of kept tiles are within tol / 2 (see the small-scene test above); the
+    pixels between them are not compared, and the escapes are what this
+    measures: the network's errors sit at single pixels (samples falling on
+    the other side of a sheet edge than the reference's), which a sparse
+    check cannot bound."""
     w, sc, exact, _, sur, _ = c5_reduced
     tol = 1e-3
     img, rs = sur.render(w.angles, 128, w.ds, 2 * w.ds, tol, integrator="simple")
     err = np.abs(img.astype(np.float64) - exact)
-    print(f"full image max {err.max():.3e}; tiles {rs['tiles']} re-rendered "
-          f"{rs['tiles_failed']}; {rs}", flush=True)
-    assert float(err.max()) <= tol
-    pure, _ = sur.render(w.angles, 128, w.ds, 2 * w.ds, 0.0, integrator="simple")
-    perr = np.abs(pure.astype(np.float64) - exact)
-    print(f"pure surrogate max {perr.max():.3e}, pixels above tol {(perr > tol).mean():.3%}",
-          flush=True)
+    print(f"sampled: full image max {err.max():.3e}, pixels above tol {(err > tol).mean():.3%}; "
+          f"tiles re-rendered {rs['tiles_failed']}/{rs['tiles']}", flush=True)
+    assert rs["check_pixels"] == 16 * rs["tiles"]
+    assert (err > tol).mean() < 0.05
 
 
 def test_surrogate_ab_against_exact_only(c5_reduced):
-    """A/B of the surrogate form (network march + exact check + fallback)
+    """A/B of the surrogate form (network march + certified exact check)
     against the exact-only march: time and samples evaluated by each, and
-    what training buys: the trained network sends fewer tiles to the exact
-    fallback than an untrained one, and both results meet the tolerance."""
+    what training buys (error and tiles kept against an untrained network);
+    both results meet the tolerance."""
     w, sc, exact, est, sur, untrained = c5_reduced
     tol = 1e-3
-    img_t, st_t = sur.render(w.angles, 128, w.ds, 2 * w.ds, tol, integrator="simple")
-    img_u, st_u = untrained.render(w.angles, 128, w.ds, 2 * w.ds, tol, integrator="simple")
+    img_t, st_t = sur.render(w.angles, 128, w.ds, 2 * w.ds, -tol, integrator="simple")
+    img_u, st_u = untrained.render(w.angles, 128, w.ds, 2 * w.ds, -tol, integrator="simple")
     exact_eval = est["samples"] - est["skipped_samples"]
     for name, st in (("trained", st_t), ("untrained", st_u)):
         ms = st["surrogate_ms"] + st["check_ms"] + st["fallback_ms"]
@@ -183,6 +206,36 @@ def test_surrogate_ab_against_exact_only(c5_reduced):
           f"(+{est['skipped_samples']} settled)", flush=True)
     assert float(np.abs(img_t.astype(np.float64) - exact).max()) <= tol
     assert float(np.abs(img_u.astype(np.float64) - exact).max()) <= tol
-    assert st_t["tiles_failed"] < st_u["tiles_failed"]
-    assert st_t["exact_samples"] < st_u["exact_samples"]
     assert st_t["network_samples"] == st_u["network_samples"] > 0
+    # What training buys.  At 1e-3 neither network keeps a tile that holds
+    # part of the object (the sheets' edges put single pixels beyond it, see
+    # test_discretization_floor_of_a_field_integrating_surrogate); the
+    # trained network's error is a small fraction of the untrained one's,
+    # and at a looser bar it keeps tiles the untrained network loses.
+    assert st_t["max_sampled_error"] < 0.1 * st_u["max_sampled_error"]
+    loose = 3e-2
+    _, lt = sur.render(w.angles, 128, w.ds, 2 * w.ds, -loose, integrator="simple")
+    _, lu = untrained.render(w.angles, 128, w.ds, 2 * w.ds, -loose, integrator="simple")
+    print(f"at {loose:g}: tiles taking exact values, trained {lt['tiles_failed']}, "
+          f"untrained {lu['tiles_failed']} of {lt['tiles']}", flush=True)
+    assert lt["tiles_failed"] < lu["tiles_failed"]
+
+
+def test_discretization_floor_of_a_field_integrating_surrogate(ctx):
+    """Why the surrogate form cannot be much faster than the plain form at
+    1e-3: the reference's image is a left-endpoint sum at ds, and the same
+    scene's exact line integral (approached here by the exact march at
+    ds / 4) differs from it by far more than 1e-3 at the thin sheets' edges.
+    A surrogate that integrates a density field anywhere but at the
+    reference's own sample points inherits that difference; staying within
+    1e-3 means classifying the reference's samples one by one, which the
+    exact gyroid test does at a fraction of a network evaluation's cost."""
+    w = W.gyroid_lattice(resolution=96, n_views=2, ds=0.001, shear=True)
+    sc = xrp_scene(w.obj, w.deformation)
+    at_ds, _ = X.render(ctx, sc, w.angles, 96, w.ds, integrator="simple")
+    fine, _ = X.render(ctx, sc, w.angles, 96, w.ds / 4, integrator="simple")
+    d = np.abs(at_ds.astype(np.float64) - fine)
+    print(f"|sum at ds - sum at ds/4|: max {d.max():.3e}, mean {d.mean():.3e}, "
+          f"pixels above 1e-3 {(d > 1e-3).mean():.2%}", flush=True)
+    assert d.max() > 1e-3
+    sc.close()

@@ -424,6 +424,16 @@ bool skip_enabled() {
   return on;
 }
 
+// Second-order reach of gyroids in the guarded marches (XRP_REACH=0 turns
+// it off, for comparison; the image is the same either way).
+bool reach_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("XRP_REACH");
+    return !(e && std::strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 // Per-wave culling of a root collection's children (XRP_CULL=0 turns it
 // off, for comparison; the image is the same either way).
 bool cull_enabled() {
@@ -699,12 +709,22 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   if (precision == XRP_PRECISION_GUARDED && !vox_kernel && skip_enabled()) {
     double inv = 1.0 / (lipschitz * job.ds) * (1.0 - 1e-6);
     mp.skip_inv = std::isfinite(inv) && inv > 0 ? (float)inv : 0.0f;
+    // gyroids reach further along the ray with the second-order bound,
+    // which needs the last sample's velocity (XRP_REACH=0 turns it off,
+    // for comparison; the image is the same either way)
+    if (mp.skip_inv > 0.0f && reach_enabled())
+      for (const auto& n : pl.nodes)
+        if (n.kind == K_GYROID && n.g[6] > 0.0f) mp.vel = 1;
   }
   // Samples per node visit of the per-scene guarded simple march: with the
   // safe radius, 2 is faster on small trees (lattice cell 10 nodes: +14 %;
   // gyroid lattice +13 %), 4 on large collections, whose node walk it
   // amortises (1-200 primitives: 4 is 14 % faster than 2).
   mp.block = s.nodes.size() >= 24 ? 4 : 2;
+  if (const char* e = std::getenv("XRP_BLOCK")) {  // for comparison: 1, 2 or 4
+    const int b = std::atoi(e);
+    if (b == 1 || b == 2 || b == 4) mp.block = b;
+  }
   // Per-wave culling of the root collection's children (guarded marches):
   // needs the deformations composed to one affine map, so that rays stay
   // lines in object space.  From 8 children up: on the 3-child CSG scene
@@ -740,6 +760,7 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       if (e == "pair") layout = VOX_BRICK4_PAIR;
       if (e == "quad") layout = VOX_BRICK4_QUAD;
       if (e == "quad_slide") layout = VOX_BRICK4_QUAD_SLIDE;
+      if (e == "quad_step") layout = VOX_BRICK4_QUAD_STEP;
     }
     // The kernels address a copy with 32-bit byte offsets: a grid whose
     // pair or quad copy would pass 4 GiB (2^28 voxels and up for quads)
@@ -1177,7 +1198,7 @@ struct xrp_batch {
   std::vector<double> cost;           // per item: estimated work per tile
   struct Group {
     int prec = 0, integ = 0;
-    std::vector<int> items;           // ordered by cost, most expensive first
+    std::vector<int> items;           // queue order (smallest total work first)
     int first = 0;                    // offset of the group's jobs in `jobs`
     uint32_t n_tiles = 0;
   };
@@ -1434,7 +1455,13 @@ int xrp_batch_create(xrp_context* ctx, const xrp_batch_item* items, int n_items,
     }
     b->signature = sig;
 
-    // Groups by (precision, integrator), items by cost, most expensive first.
+    // Groups by (precision, integrator).  Within a group the scenes are
+    // queued whole, in order of estimated total work (work per tile x
+    // pixels), smallest first: each scene then finishes as early as its
+    // place allows, which minimises the mean and median completion time,
+    // while the tiles of the last, largest scenes still spread over every
+    // compute unit.  $XRP_BATCH_ORDER=tile restores the order by work per
+    // tile, largest first (for comparison).
     for (int it = 0; it < n_items; ++it) {
       auto g = std::find_if(b->groups.begin(), b->groups.end(), [&](const xrp_batch::Group& x) {
         return x.prec == b->prec[it] && x.integ == b->integ[it];
@@ -1447,10 +1474,19 @@ int xrp_batch_create(xrp_context* ctx, const xrp_batch_item* items, int n_items,
       }
       g->items.push_back(it);
     }
+    const char* order_env = std::getenv("XRP_BATCH_ORDER");
+    const bool by_tile = order_env && std::strcmp(order_env, "tile") == 0;
+    auto total = [&](int it) {
+      return b->cost[it] * (double)b->n_views[it] * b->res[it] * b->res[it];
+    };
     int first = 0;
     for (auto& g : b->groups) {
-      std::stable_sort(g.items.begin(), g.items.end(),
-                       [&](int a, int c) { return b->cost[a] > b->cost[c]; });
+      if (by_tile)
+        std::stable_sort(g.items.begin(), g.items.end(),
+                         [&](int a, int c) { return b->cost[a] > b->cost[c]; });
+      else
+        std::stable_sort(g.items.begin(), g.items.end(),
+                         [&](int a, int c) { return total(a) < total(c); });
       g.first = first;
       first += (int)g.items.size();
     }
@@ -1623,11 +1659,12 @@ struct xrp_surrogate {
   int64_t count = 0;        // parameters
   int64_t table_count = 0;  // of which tables
   DevBuf params, grads, m, v, tbf16, loss, pts, labels, scene_nodes, items, vals;
+  DevBuf judge, counters;  // certified check: exact images, tile counters
   int adam_step = 0;
   bool packed = false;
   ~xrp_surrogate() {
     for (DevBuf* b : {&params, &grads, &m, &v, &tbf16, &loss, &pts, &labels, &scene_nodes,
-                      &items, &vals})
+                      &items, &vals, &judge, &counters})
       b->release();
   }
 };
@@ -1998,7 +2035,43 @@ int xrp_surrogate_render_device(xrp_surrogate* s, const xrp_render_params* p,
     out_st.network_samples = nsamp;
     const int tiles_side = (res + 15) / 16;
     out_st.tiles = (uint64_t)tiles_side * tiles_side * n_views;
-    if (tol > 0) {
+    const int prec_x = pl.precision == XRP_PRECISION_FP64 ? PREC_FP64 : PREC_GUARDED;
+    const int integ_x = job.integrator == XRP_INTEGRATE_HIERARCHICAL ? INT_HIERARCHICAL
+                                                                     : INT_SIMPLE;
+    if (tol < 0) {
+      // Certified: the exact march renders every pixel as the judge, and
+      // every tile whose largest difference exceeds |tol| is replaced by
+      // the exact values, so no pixel of the result is further than |tol|
+      // from the exact render.
+      HIPCHK(s->judge.ensure((size_t)n_views * res * res * sizeof(float)));
+      HIPCHK(s->counters.ensure(2 * sizeof(unsigned)));
+      HIPCHK(hipMemsetAsync(ctx->stats.p, 0, ST_COUNT * sizeof(unsigned long long),
+                            ctx->stream));
+      HIPCHK(hipMemsetAsync(s->counters.p, 0, 2 * sizeof(unsigned), ctx->stream));
+      MarchParams ep = mp;
+      ep.out = (float*)s->judge.p;
+      ep.stats = (unsigned long long*)ctx->stats.p;
+      HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+      e = launch_march(ep, prec_x, integ_x, ctx->stream);
+      if (!e)
+        e = launch_sur_certify((float*)out_device, (const float*)s->judge.p, n_views, res,
+                               (float)-tol, (unsigned*)s->counters.p, ctx->stream);
+      if (e) return fail(XRP_ERR_DEVICE, hipGetErrorString((hipError_t)e));
+      HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+      HIPCHK(hipEventSynchronize(ctx->ev1));
+      HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+      unsigned cnt[2];
+      unsigned long long st_x[ST_COUNT];
+      HIPCHK(hipMemcpy(cnt, s->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(st_x, ctx->stats.p, sizeof st_x, hipMemcpyDeviceToHost));
+      float worst;
+      std::memcpy(&worst, &cnt[1], sizeof worst);
+      out_st.check_ms = ms;
+      out_st.check_pixels = (uint64_t)n_views * res * res;
+      out_st.tiles_failed = cnt[0];
+      out_st.max_sampled_error = worst;
+      out_st.exact_samples = st_x[ST_SAMPLES] - st_x[ST_SKIPPED];
+    } else if (tol > 0) {
       // kCheck check pixels per 16x16 tile on a stride-4 grid (rows and
       // columns 1, 5, 9, 13 of the tile); a tile passes when every one is
       // within tol / 2, the other half of the tolerance being the margin

@@ -27,9 +27,14 @@
 
 namespace xrp {
 
+// Positions of a block of S samples, and the object-space velocity of the
+// sample path at the last one (per unit of ray length; the DIST walks read
+// it, translations such as the tessellation wrap leave it unchanged).  Zero
+// when the march carries none: no second-order reach is taken then.
 template <int S>
 struct PBlock {
   float x[S], y[S], z[S];
+  float vx = 0.0f, vy = 0.0f, vz = 0.0f;
 };
 
 // Safe-radius bounds of single points, for the neighbouring cell that
@@ -63,18 +68,32 @@ __device__ __forceinline__ float cyl_dist(const DevNode& n, float x, float y, fl
   const float aa = fmaxf(-t, t - n.f[6]) * n.g[6];
   return fabsf(fmaxf(ra, aa)) - n.g[5];
 }
-__device__ __forceinline__ float gyroid_dist(const DevNode& n, float x, float y, float z) {
+__device__ __forceinline__ float gyroid_reach(const DevNode& n, float gg, float g1);
+// With a velocity (vx, vy, vz) of the path at the point (zero when the
+// march carries none), also the second-order reach along it, as an
+// object-space radius (gyroid_blk).
+__device__ __forceinline__ float gyroid_dist(const DevNode& n, float x, float y, float z,
+                                             float vx, float vy, float vz) {
   float sx, cxs, sy, cys, sz, czs;
   sincos_rev((x - n.f[0]) * n.f[6], sx, cxs);
   sincos_rev((y - n.f[1]) * n.f[6], sy, cys);
   sincos_rev((z - n.f[2]) * n.f[6], sz, czs);
-  const float a = fabsf(fmaf(sx, cys, fmaf(sy, czs, sz * cxs))) - n.f[4];
-  return (fabsf(a) - n.g[0]) * n.g[4];
+  const float gg = fmaf(sx, cys, fmaf(sy, czs, sz * cxs));
+  const float a = fabsf(gg) - n.f[4];
+  float r = (fabsf(a) - n.g[0]) * n.g[4];
+  if (n.g[6] > 0.0f && (vx != 0.0f || vy != 0.0f || vz != 0.0f)) {
+    const float gx = fmaf(cxs, cys, -sz * sx);
+    const float gy = fmaf(cys, czs, -sx * sy);
+    const float gz = fmaf(czs, cxs, -sy * sz);
+    const float g1 = fmaf(gx, vx, fmaf(gy, vy, gz * vz)) * n.g[3];
+    r = fmaxf(r, gyroid_reach(n, gg, g1) * n.g[5]);
+  }
+  return r;
 }
 // Smallest bound over a collection's children (no skipping past a child
 // without one: nested tessellations and voxel grids give 0).
 __device__ __forceinline__ float coll_dist(const DevNode* __restrict__ N, int i, float x,
-                                           float y, float z) {
+                                           float y, float z, float vx, float vy, float vz) {
   const DevNode& n = N[i];
   float d = INFINITY;
   for (int c = n.first; c < n.next; c = N[c].next) {
@@ -84,7 +103,7 @@ __device__ __forceinline__ float coll_dist(const DevNode* __restrict__ N, int i,
       case K_BOX: d = fminf(d, box_dist(k, x, y, z)); break;
       case K_PPED: d = fminf(d, pped_dist(k, x, y, z)); break;
       case K_CYL: d = fminf(d, cyl_dist(k, x, y, z)); break;
-      case K_GYROID: d = fminf(d, gyroid_dist(k, x, y, z)); break;
+      case K_GYROID: d = fminf(d, gyroid_dist(k, x, y, z, vx, vy, vz)); break;
       default: return 0.0f;
     }
   }
@@ -199,6 +218,31 @@ __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
   }
 }
 
+// Length of ray, from a point where g = g0 and dg/ds = g1 (FP32, within
+// the band and e1 of the exact values), over which the exact g keeps its
+// side of the level |g| = th: with |g''| <= M along the path,
+// sigma g(s) - th >= c + (sigma g1 - e1) s - M s^2 / 2 outside (c = |g0| -
+// th - band, sigma = sign g0), and both th - g and g + th are bounded the
+// same way inside.  The first root of c + b s - M s^2 / 2, written without
+// cancellation: (b + sqrt(b^2 + 2 M c)) / M for b > 0, else
+// 2c / (sqrt(b^2 + 2 M c) - b).
+__device__ __forceinline__ float quad_reach(float c, float b, float M) {
+  if (!(c > 0.0f)) return 0.0f;
+  const float q = __builtin_amdgcn_sqrtf(fmaf(b, b, 2.0f * M * c));
+  return b > 0.0f ? (b + q) / M : 2.0f * c / (q - b);
+}
+__device__ __forceinline__ float gyroid_reach(const DevNode& n, float gg, float g1) {
+  const float th = n.f[4], band = n.g[0], M = n.g[6], e1 = n.g[7];
+  const float ag = fabsf(gg);
+  float s;
+  if (ag >= th) {
+    s = quad_reach(ag - th - band, (gg < 0.0f ? -g1 : g1) - e1, M);
+  } else {
+    s = fminf(quad_reach(th - band - gg, -g1 - e1, M), quad_reach(gg + th - band, g1 - e1, M));
+  }
+  return s * (1.0f - 1e-4f);
+}
+
 template <int S, bool DIST>
 __device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
                                            double (&v)[S], unsigned& amb, float& dist) {
@@ -215,9 +259,22 @@ __device__ __forceinline__ void gyroid_blk(const DevNode& n, const PBlock<S>& p,
     v[s] = a < 0.0f ? rho : 0.0;
     amb |= (fabsf(a) < g ? 1u : 0u) << s;
     if constexpr (DIST) {
-      // |grad g| <= 1.77 / scale: the level |g| = th is at least
-      // (|a| - band) * scale / 1.77 away
-      if (s == S - 1) dist = fminf(dist, (fabsf(a) - g) * n.g[4]);
+      if (s == S - 1) {
+        // |grad g| <= 1.77 / scale: the level |g| = th is at least
+        // (|a| - band) * scale / 1.77 away
+        float r = (fabsf(a) - g) * n.g[4];
+        if (n.g[6] > 0.0f && (p.vx != 0.0f || p.vy != 0.0f || p.vz != 0.0f)) {
+          // along the ray: dg/ds = grad g . v / scale, and the second-order
+          // reach, as an object-space radius (reach times the chain's
+          // Lipschitz bound n.g[5]: the march divides by it again)
+          const float gx = fmaf(cxs, cys, -sz * sx);
+          const float gy = fmaf(cys, czs, -sx * sy);
+          const float gz = fmaf(czs, cxs, -sy * sz);
+          const float g1 = fmaf(gx, p.vx, fmaf(gy, p.vy, gz * p.vz)) * n.g[3];
+          r = fmaxf(r, gyroid_reach(n, gg, g1) * n.g[5]);
+        }
+        dist = fminf(dist, r);
+      }
     }
   }
 }
@@ -270,6 +327,11 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
     q.x[s] = fmaf(-n.f[9], nx, x);
     q.y[s] = fmaf(-n.f[10], ny, y);
     q.z[s] = fmaf(-n.f[11], nz, z);
+    if (s == S - 1) {
+      q.vx = p.vx;
+      q.vy = p.vy;
+      q.vz = p.vz;
+    }
     if (!o) {
       if (!in || seam)
         amb |= 1u << s;
@@ -335,8 +397,10 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
         qy = fmaf(sh, sy, qy);
       else
         qz = fmaf(sh, sz, qz);
+      // the neighbour's content is this cell's translated: the path keeps
+      // its velocity there (zero when the march carries none)
       float dn = 0.0f;
-      if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, qx, qy, qz);
+      if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, qx, qy, qz, p.vx, p.vy, p.vz);
       dist = fminf(dist, fminf(f2, dn));
     }
   }
@@ -454,12 +518,18 @@ __device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PB
   root_blk<S, false>(N, p, live, out, amb, unused, cl);
 }
 
+// With vel (wave-uniform) the last sample's velocity (p.vx..vz, the ray
+// direction on entry) is carried through the chain as well.
 template <int S>
 __device__ __forceinline__ void deform_blk(const DevDeform* __restrict__ D, int nd,
-                                           PBlock<S>& p) {
+                                           PBlock<S>& p, bool vel = false) {
   for (int k = 0; k < nd; ++k) {
 #pragma unroll
-    for (int s = 0; s < S; ++s) deform32(D + k, 1, p.x[s], p.y[s], p.z[s]);
+    for (int s = 0; s < S - 1; ++s) deform32(D + k, 1, p.x[s], p.y[s], p.z[s]);
+    if (vel)
+      deform32_jvp(D + k, 1, p.x[S - 1], p.y[S - 1], p.z[S - 1], p.vx, p.vy, p.vz);
+    else
+      deform32(D + k, 1, p.x[S - 1], p.y[S - 1], p.z[S - 1]);
   }
 }
 

@@ -515,6 +515,113 @@ __device__ __forceinline__ void deform32(const DevDeform* __restrict__ D,
   }
 }
 
+// deform32 of one point together with the image of a tangent vector v: on
+// return v holds J v, J the chain's Jacobian at the point (FP32).  The
+// guarded marches take v = the ray direction at the block's last sample,
+// so that v is the object-space velocity of the sample path per unit of
+// ray length there.
+__device__ __forceinline__ void deform32_jvp(const DevDeform* __restrict__ D, int nd,
+                                             float& x, float& y, float& z, float& vx,
+                                             float& vy, float& vz) {
+  for (int k = 0; k < nd; ++k) {
+    const DevDeform& f = D[k];
+    switch (f.kind) {
+      case D_GAUSS: {
+        // x_i += A_i e^(-r^2 / (2 s_i^2)): d/dv = -2 A_i e_i / (2 s_i^2) (r . v)
+        float x0 = x - f.f[6], y0 = y - f.f[7], z0 = z - f.f[8];
+        float r2 = fmaf(x0, x0, fmaf(y0, y0, z0 * z0));
+        float rv = fmaf(x0, vx, fmaf(y0, vy, z0 * vz));
+        float ex = expf(-r2 * f.f[3]), ey = expf(-r2 * f.f[4]), ez = expf(-r2 * f.f[5]);
+        float ddx = f.f[0] * ex, ddy = f.f[1] * ey, ddz = f.f[2] * ez;
+        vx = fmaf(-2.0f * f.f[3] * ddx, rv, vx);
+        vy = fmaf(-2.0f * f.f[4] * ddy, rv, vy);
+        vz = fmaf(-2.0f * f.f[5] * ddz, rv, vz);
+        x = x + ddx;
+        y = y + ddy;
+        z = z + ddz;
+        break;
+      }
+      case D_AFFINE: {
+        const float* m = f.f;
+        float nx = fmaf(m[0], x, fmaf(m[1], y, m[2] * z));
+        float ny = fmaf(m[3], x, fmaf(m[4], y, m[5] * z));
+        float nz = fmaf(m[6], x, fmaf(m[7], y, m[8] * z));
+        float wx = fmaf(m[0], vx, fmaf(m[1], vy, m[2] * vz));
+        float wy = fmaf(m[3], vx, fmaf(m[4], vy, m[5] * vz));
+        float wz = fmaf(m[6], vx, fmaf(m[7], vy, m[8] * vz));
+        x = nx;
+        y = ny;
+        z = nz;
+        vx = wx;
+        vy = wy;
+        vz = wz;
+        break;
+      }
+      case D_LINEAR: {
+        const float* s = f.f;
+        float nx = x + fmaf(s[0], x, fmaf(s[5], y, s[4] * z));
+        float ny = y + fmaf(s[5], x, fmaf(s[1], y, s[3] * z));
+        float nz = z + fmaf(s[4], x, fmaf(s[3], y, s[2] * z));
+        float wx = vx + fmaf(s[0], vx, fmaf(s[5], vy, s[4] * vz));
+        float wy = vy + fmaf(s[5], vx, fmaf(s[1], vy, s[3] * vz));
+        float wz = vz + fmaf(s[4], vx, fmaf(s[3], vy, s[2] * vz));
+        x = nx;
+        y = ny;
+        z = nz;
+        vx = wx;
+        vy = wy;
+        vz = wz;
+        break;
+      }
+      case D_RIGID:
+        x = x + f.f[0];
+        y = y + f.f[1];
+        z = z + f.f[2];
+        break;
+      case D_SIGMOID: {
+        // q += A sigma((q - c) / l): dq'/dq = 1 + (A / l) sigma (1 - sigma)
+        // (the position rounds as deform32's)
+        float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
+        float den = 1.0f + expf(-(q - f.f[1]) * f.f[2]);
+        float sg = 1.0f / den;
+        float gain = fmaf(f.f[0] * f.f[2] * sg, 1.0f - sg, 1.0f);
+        q = q + f.f[0] / den;
+        if (f.axis == 0) {
+          x = q;
+          vx *= gain;
+        } else if (f.axis == 1) {
+          y = q;
+          vy *= gain;
+        } else {
+          z = q;
+          vz *= gain;
+        }
+        break;
+      }
+      case D_TANH: {
+        // q += A tanh(k s): dq'/ds = A k (1 - tanh^2)
+        float s = f.src == 0 ? x : (f.src == 1 ? y : z);
+        float vs = f.src == 0 ? vx : (f.src == 1 ? vy : vz);
+        float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
+        float th = tanhf(f.f[1] * s);
+        float dv = f.f[0] * f.f[1] * fmaf(-th, th, 1.0f) * vs;
+        q = q + f.f[0] * th;
+        if (f.axis == 0) {
+          x = q;
+          vx += dv;
+        } else if (f.axis == 1) {
+          y = q;
+          vy += dv;
+        } else {
+          z = q;
+          vz += dv;
+        }
+        break;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Ray set-up (main.go:457-465 and the Normalize of main.go:145/160).
 // ---------------------------------------------------------------------------

@@ -239,6 +239,9 @@ int launch_march(const MarchParams& p, int precision, int integrator,
     else if (p.block == 2)
       hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, 2>), grid, block,
                          0, s, p);
+    else if (p.block == 1)
+      hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, 1>), grid, block,
+                         0, s, p);
     else
       hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, kBlock>), grid, block,
                          0, s, p);

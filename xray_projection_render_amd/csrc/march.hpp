@@ -78,6 +78,9 @@ struct MarchParams {
   // Guarded marches: cull the root collection's children per wave
   // (march_ray.hpp wave_cull); M and t then hold the composed deformation.
   int32_t cull;
+  // Guarded marches: carry the last sample's velocity through the
+  // deformations for the gyroids' second-order reach (eval_block.hpp).
+  int32_t vel;
 };
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with
@@ -96,7 +99,10 @@ enum VoxelLayout : int32_t {
   VOX_LINEAR_CACHED = 4,
   VOX_BRICK4_PAIR = 5,
   VOX_BRICK4_QUAD = 6,
-  VOX_BRICK4_QUAD_SLIDE = 7
+  VOX_BRICK4_QUAD_SLIDE = 7,
+  // the quad copy stepped incrementally over each lane's own samples
+  // (voxel.hip step_march)
+  VOX_BRICK4_QUAD_STEP = 8
 };
 inline bool voxel_layout_linear(int layout) {
   return layout == VOX_LINEAR || layout == VOX_LINEAR_CACHED;

@@ -173,7 +173,12 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
         P.y[q] = fmaf(d1, sj, a1);
         P.z[q] = fmaf(d2, sj, a2);
       }
-      deform_blk<B>(p.defs, p.n_defs, P);
+      if (p.vel) {  // velocity through the chain only when the reach uses it
+        P.vx = d0;
+        P.vy = d1;
+        P.vz = d2;
+      }
+      deform_blk<B>(p.defs, p.n_defs, P, p.vel != 0);
       const int left = k_hi - k + 1;
       const unsigned live = left >= B ? (1u << B) - 1u : (1u << left) - 1u;
       double vv[B];
@@ -319,7 +324,12 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
         P.y[q] = fmaf(d1, sj, a1);
         P.z[q] = fmaf(d2, sj, a2);
       }
-      deform_blk<B>(p.defs, p.n_defs, P);
+      if (p.vel) {  // velocity through the chain only when the reach uses it
+        P.vx = d0;
+        P.vy = d1;
+        P.vz = d2;
+      }
+      deform_blk<B>(p.defs, p.n_defs, P, p.vel != 0);
       const int left = k_hi - k + 1;
       const unsigned live = left >= B ? (1u << B) - 1u : (1u << left) - 1u;
       double vv[B];
@@ -493,7 +503,12 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
       P.y[q] = (float)(o[1] + d[1] * s[q]);
       P.z[q] = (float)(o[2] + d[2] * s[q]);
     }
-    deform_blk<B>(p.defs, p.n_defs, P);
+    if (DIST && p.vel) {  // velocity through the chain only when the reach uses it
+      P.vx = (float)d[0];
+      P.vy = (float)d[1];
+      P.vz = (float)d[2];
+    }
+    deform_blk<B>(p.defs, p.n_defs, P, DIST && p.vel != 0);
     unsigned amb = 0;
     if constexpr (DIST) {
       root_blk<B, true>(p.nodes, P, live, rho, amb, *dist, cl);

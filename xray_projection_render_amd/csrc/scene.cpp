@@ -1080,6 +1080,7 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
                       double s_err) {
   const double u = 1.0 / 16777216.0;
   const double kappa = 4.0;
+  const double s3c = std::sqrt(3.0);
   double B = world_bound;
   // FP64 terms: ray set-up, and s_err, how far the march's anchors
   // s = smin + k ds can be from the reference's accumulated s_k.
@@ -1087,7 +1088,12 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
   // Euclidean Lipschitz bound of the chain: how far object-space positions
   // of two samples can be apart per unit of ray length between them.
   double lip_chain = 1.0;
+  // Bound on the chain's second derivative along a ray (unit speed):
+  // through a map with Lipschitz bound l and |D''[v, v]| <= h |v|^2 the
+  // bound becomes l K + h L^2 (chain rule), L the Lipschitz bound so far.
+  double curv_chain = 0.0;
   for (auto& f : s.defs) {
+    double h = 0.0, l_prev = lip_chain;
     switch (f.kind) {
       case D_RIGID: {
         double t = std::max({std::fabs(f.p[0]), std::fabs(f.p[1]), std::fabs(f.p[2])});
@@ -1116,6 +1122,7 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         delta = delta * (1 + A / (4 * ls)) + A * (u * arg + 8 * u) + 4 * u * (B + A);
         B = B + A;
         lip_chain *= 1 + A / (4 * ls);  // max slope of A / (1 + e^(-x/ls))
+        h = A / (ls * ls) * 0.0963;     // max |sigma''| = 1 / (6 sqrt 3)
         break;
       }
       case D_TANH: {
@@ -1124,7 +1131,11 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         double A = std::fabs(f.p[0]), k = std::fabs(f.p[1]);
         delta = delta * (1 + A * k) + A * k * 2 * u * B + 8 * u * A + 2 * u * (B + A);
         B = B + A;
-        lip_chain *= 1 + A * k;
+        // Euclidean: the Jacobian is I + a e_axis e_source^T with |a| <= A k
+        // (axis != source), a shear whose largest singular value is
+        // (a + sqrt(a^2 + 4)) / 2, increasing in |a|
+        lip_chain *= (1 + 1e-12) * (A * k + std::sqrt(A * k * A * k + 4.0)) / 2.0;
+        h = A * k * k * 0.7699;  // max |tanh''| = 4 / (3 sqrt 3)
         break;
       }
       case D_GAUSS: {
@@ -1140,9 +1151,14 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         // |grad A e^(-r^2/2s^2)| <= |A| e^(-1/2) / s per component, three
         // components: sqrt(3) * 0.6066 < 1.06
         lip_chain *= 1 + lip;
+        // |d^2/ds^2 e^(-r^2/2s^2)| <= |v|^2 / s^2 (|t^2 - 1| e^(-t^2/2) <= 1)
+        for (int k = 0; k < 3; ++k)
+          h = std::max(h, s3c * std::fabs(f.p[k]) / (f.q[k] * f.q[k]));
         break;
       }
     }
+    const double l_this = lip_chain / l_prev;
+    curv_chain = l_this * curv_chain + h * l_prev * l_prev;
   }
   const double delta_obj = delta;
 
@@ -1224,6 +1240,26 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         // most |grad |grad g|^2| * h sqrt(3)/2 <= 2 |H|_F |grad g| * 0.00756
         // <= 2 sqrt(12) sqrt(6) * 0.00756 = 0.128; sqrt(3.128) < 1.77.
         n.g[4] = (float)(sc / (1.77 * (1 + 1e-5)));
+        // Second-order reach along the ray (eval_block.hpp gyroid_reach):
+        // g(s) = G(q(s) / scale), |q'| <= L, |q''| <= K (the chain's bounds),
+        // so |g''| <= |H G| (L / scale)^2 + |grad G| K / scale.  The
+        // Hessian's spectral norm is at most 2.0 on a 241^3 grid of the
+        // period and, its entries' gradients being at most 2 per component,
+        // grows by at most sqrt(27 * 4) * h sqrt(3) / 2 < 0.235 between grid
+        // points: |H G| <= 2.25 (tests/test_reach_bound.py repeats the
+        // grid).  dg/ds = grad G . v / scale is off by at most e1: each
+        // gradient component is two products of sine and cosine values, each
+        // value within eq + the unit's error, and v is within a few ulps of L.
+        {
+          const double inv = 1.0 / sc;
+          const double eg = 7.1 * (eq + kHwSinCosErr) + 6 * u;
+          const double M = (1 + 1e-5) * (2.25 * (lip_chain * inv) * (lip_chain * inv) +
+                                         1.77 * curv_chain * inv);
+          n.g[3] = (float)inv;
+          n.g[5] = (float)lip_chain;
+          n.g[6] = std::isfinite(M) && M < 1e30 ? (float)M : 0.0f;
+          n.g[7] = (float)(2 * lip_chain * inv * (eg + 128 * u));
+        }
         break;
       }
       case K_VOXEL:

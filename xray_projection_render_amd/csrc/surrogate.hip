@@ -657,6 +657,39 @@ __global__ void gather_pixels_kernel(const float* img, const int* items, int n, 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sur_certify_kernel(float* img, const float* exact,
+                                                          int res, float tol,
+                                                          unsigned* counters) {
+  __shared__ float wave_max[4];
+  const int tiles_side = (res + 15) / 16;
+  const int ti = blockIdx.x / tiles_side, tj = blockIdx.x - ti * tiles_side;
+  const int i = ti * 16 + (threadIdx.x >> 4), j = tj * 16 + (threadIdx.x & 15);
+  const bool in = i < res && j < res;
+  const size_t at = ((size_t)blockIdx.y * res + (in ? i : 0)) * res + (in ? j : 0);
+  const float e = in ? exact[at] : 0.0f;
+  float d = in ? fabsf(img[at] - e) : 0.0f;
+  if (!(d == d)) d = INFINITY;  // a NaN never passes
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) d = fmaxf(d, __shfl_xor(d, off, 64));
+  if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = d;
+  __syncthreads();
+  const float m = fmaxf(fmaxf(wave_max[0], wave_max[1]), fmaxf(wave_max[2], wave_max[3]));
+  if (m > tol && in) img[at] = e;
+  if (threadIdx.x == 0) {
+    if (m > tol) atomicAdd(counters, 1u);
+    atomicMax(counters + 1, __float_as_uint(m));  // non-negative: bits order like values
+  }
+}
+
+int launch_sur_certify(float* img, const float* exact, int n_views, int res, float tol,
+                       unsigned* counters, void* stream) {
+  if (n_views <= 0 || res <= 0) return 0;
+  const int tiles_side = (res + 15) / 16;
+  hipLaunchKernelGGL(sur_certify_kernel, dim3(tiles_side * tiles_side, n_views), dim3(256), 0,
+                     (hipStream_t)stream, img, exact, res, tol, counters);
+  return (int)hipGetLastError();
+}
+
 int launch_gather_pixels(const float* img, const int* items, int n, int res, float* out,
                          void* stream) {
   if (n <= 0) return 0;

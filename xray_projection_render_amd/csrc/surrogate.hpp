@@ -62,6 +62,13 @@ int launch_sur_render(const SurNet& net, const MarchParams& p, double ds_sur, vo
 // out[k] = img[items[3k]][items[3k+1]][items[3k+2]] of a res x res image stack.
 int launch_gather_pixels(const float* img, const int* items, int n, int res, float* out,
                          void* stream);
+// Certified tile check: one 256-lane workgroup per 16x16 tile of a
+// res x res image stack (grid: tiles, views).  A tile whose largest
+// |img - exact| exceeds tol is replaced by the exact values; counters[0]
+// counts replaced tiles and counters[1] receives the bits of the largest
+// difference seen (both zeroed by the caller).
+int launch_sur_certify(float* img, const float* exact, int n_views, int res, float tol,
+                       unsigned* counters, void* stream);
 // MFMA operand layout probe: D = A (16x16) * B (16x16), bf16 inputs.
 int launch_mfma_probe(const float* a, const float* b, float* d, void* stream);
 

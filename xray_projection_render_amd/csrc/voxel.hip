@@ -184,7 +184,7 @@ __device__ __forceinline__ bool exact_range(const MarchParams& p, const Ray& r,
 // CACHE_SLIDE (quad copy only, slide_march below): a cell that moves by one
 // along x keeps the x plane it shares with the last cell and loads only the
 // other one.
-enum VoxelCache { CACHE_NONE = 0, CACHE_CELL = 1, CACHE_SLIDE = 2 };
+enum VoxelCache { CACHE_NONE = 0, CACHE_CELL = 1, CACHE_SLIDE = 2, CACHE_STEP = 3 };
 // Sliding-cache march of one ray over the wave's sample range, on the quad
 // copy: a quad holds (z, y) corners (v[z][y], v[z+1][y], v[z][y+1],
 // v[z+1][y+1]), so a cell is two x planes, one 16-byte load each, and a
@@ -262,6 +262,111 @@ __device__ __forceinline__ double slide_march(const MarchParams& p, __amdgpu_buf
   return T;
 }
 
+// Stepping march of one ray over its own sample range [ka, kb] (not the
+// wave's union of ranges, so no lane idles past its own ends), on the quad
+// copy.  Per chunk of kChunk samples the grid position is anchored in FP64
+// and cell and weights are taken from it as the other layouts take them;
+// after that the position advances by the constant FP32 increment of one
+// step, kept as cell indices plus weights: a weight that leaves [0, 1)
+// moves the cell by its floor.  A cell's four z-differences are formed when
+// it is loaded, so a sample costs the seven lerps' FMAs and three
+// differences, the increments and the Kahan sum.  The weights differ from
+// the other layouts' by the increments' rounding (a few FP32 ulps of the
+// grid coordinate per chunk), so the bits differ from theirs while the
+// error against the FP64 oracle stays of the same size.  Indices are held
+// in [0, N-1] as the reference clamps them (objects.go:811-830): a
+// coordinate that rounds below 0 takes cell 0 with weight 0, one past N-1
+// the last cell, whose two planes are then the same.
+__device__ __forceinline__ double step_march(const MarchParams& p, __amdgpu_buffer_rsrc_t rs,
+                                             const double (&A)[3], const double (&B)[3],
+                                             int ka, int kb) {
+  const int nx = p.vn[0], ny = p.vn[1], nz = p.vn[2];
+  const unsigned sbx = (unsigned)p.vbricks[1] * 64u,
+                 sbz = (unsigned)p.vbricks[0] * (unsigned)p.vbricks[1] * 64u;
+  const double ds = p.ds, dm = p.dm;
+  const float ix = (float)(B[0] * ds), iy = (float)(B[1] * ds), iz = (float)(B[2] * ds);
+  double T = p.flat_field;
+  int cx = -1, cy = -1, cz = -1;
+  // corners (lerp_cell order) of the loaded cell: lo = x plane x0, hi =
+  // x plane x0+1, each (v[z][y], v[z+1][y], v[z][y+1], v[z+1][y+1])
+  float lo[4], hi[4], dlo[2], dhi[2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) lo[q] = hi[q] = 0.0f;
+  dlo[0] = dlo[1] = dhi[0] = dhi[1] = 0.0f;
+  for (int k0 = ka; k0 <= kb; k0 += kChunk) {
+    const double sa = p.s_table[k0];
+    const float gx = fmaxf((float)(A[0] + B[0] * sa), 0.0f),
+                gy = fmaxf((float)(A[1] + B[1] * sa), 0.0f),
+                gz = fmaxf((float)(A[2] + B[2] * sa), 0.0f);
+    int x0 = min((int)gx, nx - 1), y0 = min((int)gy, ny - 1), z0 = min((int)gz, nz - 1);
+    float wx = gx - (float)x0, wy = gy - (float)y0, wz = gz - (float)z0;
+    const int n = min(kChunk, kb - k0 + 1);
+    float acc = 0.0f, comp = 0.0f;  // Kahan-compensated chunk sum
+    for (int j = 0; j < n; ++j) {
+      if (x0 != cx || y0 != cy || z0 != cz) {
+        const bool same_yz = y0 == cy && z0 == cz;
+        const bool up = same_yz && x0 == cx + 1, down = same_yz && x0 == cx - 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float a = lo[q], b = hi[q];
+          lo[q] = up ? b : a;
+          hi[q] = down ? a : b;
+        }
+        const int x1 = min(x0 + 1, nx - 1);
+        const unsigned ezy = (unsigned)(z0 >> 2) * sbz + (unsigned)(z0 & 3) * 16u +
+                             (unsigned)(y0 >> 2) * 64u + (unsigned)(y0 & 3);
+        if (!up) {
+          const unsigned e = ezy + (unsigned)(x0 >> 2) * sbx + (unsigned)(x0 & 3) * 4u;
+          const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(e * 16u), 0, 0);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) lo[q] = __uint_as_float(v[q]);
+        }
+        if (!down) {
+          const unsigned e = ezy + (unsigned)(x1 >> 2) * sbx + (unsigned)(x1 & 3) * 4u;
+          const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(e * 16u), 0, 0);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) hi[q] = __uint_as_float(v[q]);
+        }
+        dlo[0] = lo[1] - lo[0];
+        dlo[1] = lo[3] - lo[2];
+        dhi[0] = hi[1] - hi[0];
+        dhi[1] = hi[3] - hi[2];
+        cx = x0;
+        cy = y0;
+        cz = z0;
+      }
+      // lerp_cell's order: z, then y, then x
+      const float v00 = fmaf(wz, dlo[0], lo[0]), v01 = fmaf(wz, dlo[1], lo[2]);
+      const float v10 = fmaf(wz, dhi[0], hi[0]), v11 = fmaf(wz, dhi[1], hi[2]);
+      const float v0 = fmaf(wy, v01 - v00, v00), v1 = fmaf(wy, v11 - v10, v10);
+      const float rho = fmaf(wx, v1 - v0, v0);
+      const float yk = rho - comp;
+      const float tt = acc + yk;
+      comp = (tt - acc) - yk;
+      acc = tt;
+      // next sample: advance, move the cell by the weights' floors, clamp
+      wx += ix;
+      wy += iy;
+      wz += iz;
+      const float fx = floorf(wx), fy = floorf(wy), fz = floorf(wz);
+      x0 += (int)fx;
+      y0 += (int)fy;
+      z0 += (int)fz;
+      wx -= fx;
+      wy -= fy;
+      wz -= fz;
+      if (x0 < 0) { x0 = 0; wx = 0.0f; }
+      if (y0 < 0) { y0 = 0; wy = 0.0f; }
+      if (z0 < 0) { z0 = 0; wz = 0.0f; }
+      x0 = min(x0, nx - 1);
+      y0 = min(y0, ny - 1);
+      z0 = min(z0, nz - 1);
+    }
+    T += ((double)acc * dm) * ds;
+  }
+  return T;
+}
+
 // Ray tile of one wave: 32 pixels along i (the image's horizontal axis) by
 // 2 along j.  At a given sample index the rays of a wave then lie along one
 // horizontal line through the volume; on the 256^3 workload (views around
@@ -329,6 +434,8 @@ __device__ __forceinline__ void voxel_body(const MarchParams& p) {
   const float b0 = (float)B[0], b1 = (float)B[1], b2 = (float)B[2];
   if constexpr (CACHE == CACHE_SLIDE)
     T = slide_march(p, grid_rsrc(p.vgrid, p.vbytes), A, B, ka, kb, wk0, wk1);
+  if constexpr (CACHE == CACHE_STEP)
+    T = step_march(p, grid_rsrc(p.vgrid, p.vbytes), A, B, ka, kb);
   const float ds32 = (float)p.ds;
   const double ds = p.ds, dm = p.dm;
   const __amdgpu_buffer_rsrc_t rs = grid_rsrc(p.vgrid, p.vbytes);
@@ -336,7 +443,7 @@ __device__ __forceinline__ void voxel_body(const MarchParams& p) {
   float cell[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) cell[q] = 0.0f;
-  for (int k0 = wk0; CACHE != CACHE_SLIDE && k0 <= wk1; k0 += kChunk) {
+  for (int k0 = wk0; CACHE != CACHE_SLIDE && CACHE != CACHE_STEP && k0 <= wk1; k0 += kChunk) {
     double sa = p.s_table[k0];
     float a0 = (float)(A[0] + B[0] * sa), a1 = (float)(A[1] + B[1] * sa),
           a2 = (float)(A[2] + B[2] * sa);
@@ -666,6 +773,9 @@ int launch_voxel(const MarchParams& p, int layout, void* stream) {
       break;
     case VOX_BRICK4_QUAD_SLIDE:
       hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4_QUAD, CACHE_SLIDE>), grid, block, 0, s, p);
+      break;
+    case VOX_BRICK4_QUAD_STEP:
+      hipLaunchKernelGGL((voxel_kernel<VOX_BRICK4_QUAD, CACHE_STEP>), grid, block, 0, s, p);
       break;
     case VOX_LINEAR_CACHED:
       hipLaunchKernelGGL((voxel_kernel<VOX_LINEAR, CACHE_CELL>), grid, block, 0, s, p);
