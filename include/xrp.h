@@ -124,7 +124,8 @@ typedef struct {
   double kernel_ms;        /* device time of the render kernels */
   double position_error;   /* FP32 object-space position error bound used */
   int32_t precision_used;  /* xrp_precision actually run */
-  int32_t reserved;
+  int32_t block;           /* samples per node visit of the guarded simple march
+                              (1, 2 or 4); 0 for the other kernels */
   uint64_t staged_chunks;   /* voxel kernel: sample chunks served from LDS */
   uint64_t staged_voxels;   /* voxel kernel: voxels copied into LDS */
   uint64_t fallback_chunks; /* voxel kernel: chunks gathered from global memory */

@@ -415,3 +415,43 @@ def test_gyroid_lattice_tanh_shear_reduced(ctx):
         assert err <= TOL
         assert st["audit_mismatch"] == 0
         assert ref.min() < 0.95
+
+
+@pytest.mark.parametrize("integrator", ["simple", "hierarchical"])
+def test_tile_orientation_changes_no_bits(ctx, monkeypatch, integrator):
+    """Rays of a wave as 2 x 32 pixels (XRP_TILE=2x32, read per render)
+    instead of 32 x 2: the same per-ray march, so the same bits."""
+    cases = [W.csg(resolution=70, n_views=2), W.lattice(resolution=48, n_views=1),
+             W.gyroid_lattice(resolution=40, n_views=1, ds=0.002, shear=True)]
+    for w in cases:
+        sc = xrp_scene(w.obj, w.deformation)
+        ds = w.ds if integrator == "simple" else 10 * w.ds
+        monkeypatch.setenv("XRP_TILE", "32x2")
+        a, _ = X.render(ctx, sc, w.angles, w.resolution, ds, integrator=integrator)
+        monkeypatch.setenv("XRP_TILE", "2x32")
+        b, _ = X.render(ctx, sc, w.angles, w.resolution, ds, integrator=integrator)
+        assert np.array_equal(a, b), w.name
+        sc.close()
+
+
+def test_block_size_changes_no_bits(ctx, monkeypatch):
+    """Samples per node visit of the guarded simple march (XRP_BLOCK, read
+    per render): 1, 2 and 4 evaluate the same samples in the same order,
+    so the images are the same bits; the render reports the block it ran
+    (1 for gyroids settled by their second-order reach, 2 for small trees)."""
+    cases = [(W.csg(resolution=70, n_views=2), 2), (W.lattice(resolution=48, n_views=1), 2),
+             (W.gyroid_lattice(resolution=40, n_views=1, ds=0.002, shear=True), 1)]
+    for w, default in cases:
+        sc = xrp_scene(w.obj, w.deformation)
+        monkeypatch.delenv("XRP_BLOCK", raising=False)
+        ref, st = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple")
+        assert st["block"] == default, w.name
+        for b in (1, 2, 4):
+            monkeypatch.setenv("XRP_BLOCK", str(b))
+            img, st = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple")
+            assert st["block"] == b
+            assert np.array_equal(img, ref), (w.name, b)
+        _, st = X.render(ctx, sc, w.angles, w.resolution, 10 * w.ds,
+                         integrator="hierarchical")
+        assert st["block"] == 0
+        sc.close()

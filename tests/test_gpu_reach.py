@@ -66,14 +66,36 @@ np.savez(sys.argv[1], **out)
 """
 
 
-@pytest.fixture(scope="module")
-def no_reach(tmp_path_factory):
+def _child(tmp_path_factory, **switches):
+    """The CASES renders made in a child process with the given switches
+    (each read once per process by the library)."""
     here = os.path.dirname(os.path.abspath(__file__))
-    path = tmp_path_factory.mktemp("reach") / "noreach.npz"
-    env = dict(os.environ, XRP_REACH="0")
+    path = tmp_path_factory.mktemp("reach") / "renders.npz"
+    env = dict(os.environ, **switches)
     subprocess.run([sys.executable, "-c", _SCRIPT % (os.path.dirname(here), here), str(path)],
                    env=env, check=True, timeout=300)
     return np.load(path)
+
+
+@pytest.fixture(scope="module")
+def no_reach(tmp_path_factory):
+    return _child(tmp_path_factory, XRP_REACH="0")
+
+
+@pytest.mark.parametrize("switches", [{"XRP_SEAMLESS": "0"}, {"XRP_QUEUE": "1"},
+                                      {"XRP_QUEUE": "0"}])
+def test_seamless_cells_and_queue_change_no_bits(ctx, tmp_path_factory, switches):
+    """Seamless tessellations (no seam guard, the radius crossing cell faces)
+    and the per-wave queue of FP64 re-evaluations change which samples are
+    evaluated, not the sum: the images equal those made without them."""
+    other = _child(tmp_path_factory, **switches)
+    for name, integ in CASES:
+        w, d = _case(name)
+        sc = xrp_scene(w.obj, d)
+        ds = w.ds if integ == "simple" else 10 * w.ds
+        img, st = X.render(ctx, sc, w.angles, w.resolution, ds, integrator=integ)
+        assert np.array_equal(img, other[name + "_" + integ]), (switches, name, integ)
+        sc.close()
 
 
 @pytest.mark.parametrize("name,integ", CASES)

@@ -76,7 +76,7 @@ class RenderStats(_OutStruct):
         ("reevaluated", ctypes.c_uint64), ("audit_mismatch", ctypes.c_uint64),
         ("fine_samples", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
         ("position_error", ctypes.c_double), ("precision_used", ctypes.c_int32),
-        ("reserved", ctypes.c_int32), ("staged_chunks", ctypes.c_uint64),
+        ("block", ctypes.c_int32), ("staged_chunks", ctypes.c_uint64),
         ("staged_voxels", ctypes.c_uint64), ("fallback_chunks", ctypes.c_uint64),
         ("skipped_samples", ctypes.c_uint64),
     ]

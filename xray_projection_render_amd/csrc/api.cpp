@@ -717,14 +717,21 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
         if (n.kind == K_GYROID && n.g[6] > 0.0f) mp.vel = 1;
   }
   // Samples per node visit of the per-scene guarded simple march: with the
-  // safe radius, 2 is faster on small trees (lattice cell 10 nodes: +14 %;
-  // gyroid lattice +13 %), 4 on large collections, whose node walk it
-  // amortises (1-200 primitives: 4 is 14 % faster than 2).
-  mp.block = s.nodes.size() >= 24 ? 4 : 2;
+  // safe radius, 2 is faster on small trees (lattice cell 10 nodes: 6 %
+  // faster than 1), 4 on large collections, whose node walk it amortises
+  // (1-200 primitives: 4 is 14 % faster than 2).  Gyroids settling by their
+  // second-order reach skip far enough that the second sample of a block is
+  // mostly wasted: 1 is 6 % faster there (6^3 sheared gyroid cells, 256 x
+  // 1024^2: 90.1 against 95.9 ms; 512^2 gyroid lattice 1.135 against 1.212).
+  mp.block = mp.vel ? 1 : s.nodes.size() >= 24 ? 4 : 2;
   if (const char* e = std::getenv("XRP_BLOCK")) {  // for comparison: 1, 2 or 4
     const int b = std::atoi(e);
     if (b == 1 || b == 2 || b == 4) mp.block = b;
   }
+  if (const char* e = std::getenv("XRP_QUEUE"))  // for comparison: 0 or 1
+    mp.queue = std::strcmp(e, "1") == 0 ? 1 : 0;
+  if (const char* e = std::getenv("XRP_TILE"))  // for comparison: 32x2 or 2x32
+    mp.tile_j32 = std::strcmp(e, "2x32") == 0 ? 1 : 0;
   // Per-wave culling of the root collection's children (guarded marches):
   // needs the deformations composed to one affine map, so that rays stay
   // lines in object space.  From 8 children up: on the 3-child CSG scene
@@ -938,6 +945,10 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     o.kernel_ms = ms;
     o.position_error = delta;
     o.precision_used = precision;
+    o.block = precision == XRP_PRECISION_GUARDED && !vox_kernel &&
+                      job.integrator == XRP_INTEGRATE_SIMPLE
+                  ? pl.mp.block
+                  : 0;
     write_out(st, o);
   }
   return XRP_OK;

@@ -75,6 +75,11 @@ static_assert(sizeof(DevNode) == 256, "DevNode must stay 256 bytes");
 //  guards: SPHERE/CYL q-band in [0]; CYL t-band in [1]; BOX/PPED per axis
 //          [0..2]; GYROID [0]; TESS wrap (fraction units) [0], bounds [1];
 //          VOXEL bounds [0]; COLL sum-near-zero band [0] (hierarchical).
+//  g[] also holds, per node kind, the safe-radius terms (eval_block.hpp):
+//          GYROID [3] 1/scale, [4] radius factor, [5] the deformation chain's
+//          Lipschitz bound, [6] the curvature bound M of the second-order
+//          reach (0: none), [7] the slope's error bound; TESS [2] 1 when the
+//          content is seamless (a whole number of gyroid periods per cell).
 
 struct alignas(16) DevDeform {
   int32_t kind;

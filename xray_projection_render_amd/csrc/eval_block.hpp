@@ -304,6 +304,11 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
   const DevNode& n = N[i];
   const float b0 = n.f[0], b1 = n.f[1], b2 = n.f[2], b3 = n.f[3], b4 = n.f[4], b5 = n.f[5];
   const float gb = n.g[1], gw = n.g[0];
+  // Seamless content (scene.cpp: every child of the unit cell a gyroid whose
+  // period divides the cell along each axis): either side of a seam wraps to
+  // the same density function, so seams need no guard and the safe radius
+  // may cross cell faces.
+  const bool seamless = n.g[2] != 0.0f;
   PBlock<S> q;
   unsigned go = 0, amb = 0;
 #pragma unroll
@@ -333,7 +338,7 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
       q.vz = p.vz;
     }
     if (!o) {
-      if (!in || seam)
+      if (!in || (seam && !seamless))
         amb |= 1u << s;
       else
         go |= 1u << s;
@@ -355,7 +360,7 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
   } else {
     amb_out |= go;
   }
-  if constexpr (DIST) {
+  if (DIST && !seamless) {
     // Last sample inside the bounds: a radius up to the nearest cell face
     // stays in this cell, whose content the walk above bounded.  Beyond it,
     // the neighbour across that face holds the same content translated, and

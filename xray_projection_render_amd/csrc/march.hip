@@ -48,18 +48,22 @@ namespace xrp {
 constexpr int march_tile_i(int) { return 32; }
 constexpr int march_tile_j(int) { return 2; }
 
-template <int PREC, int INTEG, int B>
+// Q: queue the FP64 re-evaluations per wave (always at B = 4).
+template <int PREC, int INTEG, int B, bool Q = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
-    INTEG == INT_HIERARCHICAL ? 3 : (PREC == PREC_GUARDED && B == 4 ? 3 : 4)))) void march_kernel(
-    MarchParams p) {
+    INTEG == INT_HIERARCHICAL ? 3 : (PREC == PREC_GUARDED && (B == 4 || Q) ? 3 : 4)))) void
+march_kernel(MarchParams p) {
   const int lane = threadIdx.x;
-  constexpr int TI = march_tile_i(INTEG), TJ = march_tile_j(INTEG);
+  // p.tile_j32: the transposed tile, 2 along i by 32 along j (the image's
+  // contiguous axis), so a wave writes two whole 128-byte row segments
+  const int TI = p.tile_j32 ? 2 : march_tile_i(INTEG);
+  const int TJ = p.tile_j32 ? 32 : march_tile_j(INTEG);
   const int tiles_j = (p.res + TJ - 1) / TJ;
   const int ti = blockIdx.x / tiles_j, tj = blockIdx.x - ti * tiles_j;
   const int i = ti * TI + lane / TJ;
   const int j = tj * TJ + lane % TJ;
   RayCounts rc;
-  march_ray<PREC, INTEG, 1, B, PREC == PREC_GUARDED && INTEG == INT_SIMPLE && B == 4>(
+  march_ray<PREC, INTEG, 1, B, PREC == PREC_GUARDED && INTEG == INT_SIMPLE && (B == 4 || Q)>(
       p, blockIdx.y, i, j, i < p.res && j < p.res, rc);
   flush_counts(p.stats, rc);
 }
@@ -221,7 +225,8 @@ int launch_volume(const MarchParams& p, int res, double* dens, unsigned long lon
 int launch_march(const MarchParams& p, int precision, int integrator,
                  void* stream) {
   dim3 block(64);
-  const int ti = march_tile_i(integrator), tj = march_tile_j(integrator);
+  const int ti = p.tile_j32 ? 2 : march_tile_i(integrator);
+  const int tj = p.tile_j32 ? 32 : march_tile_j(integrator);
   const int tiles = ((p.res + ti - 1) / ti) * ((p.res + tj - 1) / tj);
   dim3 grid(tiles, p.n_views);
   hipStream_t s = (hipStream_t)stream;
@@ -236,6 +241,9 @@ int launch_march(const MarchParams& p, int precision, int integrator,
     if (integrator == INT_HIERARCHICAL)
       hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_HIERARCHICAL, kBlock>), grid,
                          block, 0, s, p);
+    else if (p.block == 2 && p.queue)
+      hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, 2, true>), grid, block,
+                         0, s, p);
     else if (p.block == 2)
       hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, 2>), grid, block,
                          0, s, p);

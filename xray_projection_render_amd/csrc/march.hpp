@@ -81,6 +81,12 @@ struct MarchParams {
   // Guarded marches: carry the last sample's velocity through the
   // deformations for the gyroids' second-order reach (eval_block.hpp).
   int32_t vel;
+  // Guarded simple march at block 2: queue the FP64 re-evaluations per wave
+  // (march_ray.hpp march_T_guarded) instead of making them inline.
+  int32_t queue;
+  // Per-scene march kernel: rays of a wave as 2 x 32 pixels (32 along the
+  // image's contiguous axis j) instead of 32 x 2.
+  int32_t tile_j32;
 };
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with

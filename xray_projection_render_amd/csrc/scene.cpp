@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <limits>
@@ -1076,6 +1077,16 @@ static double spectral_norm_bound(const double* M) {
   return std::exp((log_scale + std::log(row)) / (2.0 * n)) * (1 + 1e-9);
 }
 
+// $XRP_SEAMLESS=0 treats every tessellation as seamed (for comparison; the
+// images are the same either way).
+static bool seamless_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("XRP_SEAMLESS");
+    return !(e && std::strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 double compute_guards(Scene& s, double world_bound, double step, double* lipschitz,
                       double s_err) {
   const double u = 1.0 / 16777216.0;
@@ -1161,6 +1172,33 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
     curv_chain = l_this * curv_chain + h * l_prev * l_prev;
   }
   const double delta_obj = delta;
+
+  // Seamless tessellations: every child of the unit cell's collection is a
+  // gyroid whose period 2 pi scale divides the cell along each axis (within
+  // 1e-9 of a whole number of periods), so the wrap shifts a point by whole
+  // periods and the density function is the same on both sides of a seam.
+  // Returns how far from whole periods the cell side is, as an object-space
+  // shift (the wrap into either neighbouring cell moves the gyroid's
+  // argument by at most that much more than whole periods: added to the
+  // children's position error), or -1 when the tessellation has seams.
+  auto seamless = [&](const DevNode& t) -> double {
+    if (t.first < 0 || t.first >= (int)s.nodes.size()) return -1.0;
+    const DevNode& c = s.nodes[t.first];
+    if (c.kind != K_COLL || c.first >= c.next) return -1.0;
+    double shift = 0.0;
+    for (int k = c.first; k < c.next; k = s.nodes[k].next) {
+      const DevNode& g = s.nodes[k];
+      if (g.kind != K_GYROID) return -1.0;
+      const double period = 2.0 * M_PI * std::fabs(g.d[3]);
+      for (int a = 0; a < 3; ++a) {
+        const double side = t.d[9 + a] - t.d[6 + a];
+        const double m = side / period;
+        if (!(m >= 0.5) || std::fabs(m - std::round(m)) > 1e-9 * m) return -1.0;
+        shift = std::max(shift, std::fabs(side - std::round(m) * period));
+      }
+    }
+    return std::sqrt(3.0) * shift * (1.0 + 1e-12);
+  };
 
   // Walk the node table; per-depth (delta, B) pairs.
   double dl[kMaxDepth + 2], Bl[kMaxDepth + 2];
@@ -1276,9 +1314,11 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         }
         double bb = std::min(b, bm + 1.0);  // points past the bounds return early
         n.g[1] = (float)(kappa * (d + u * (bb + bm)));
+        const double shift = seamless_enabled() ? seamless(n) : -1.0;
+        n.g[2] = shift >= 0.0 ? 1.0f : 0.0f;
         double F = (bb + um) / dxmin;
         n.g[0] = (float)(kappa * ((d + u * um + 2 * u * (bb + um)) / dxmin + 2 * u * F));
-        dl[n.depth + 1] = d + 4 * u * (bb + um + dxmax);
+        dl[n.depth + 1] = d + 4 * u * (bb + um + dxmax) + std::max(shift, 0.0);
         Bl[n.depth + 1] = um;
         break;
       }
