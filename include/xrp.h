@@ -134,6 +134,13 @@ typedef struct {
                                the nearest boundary, less every FP32 error) of an
                                evaluated sample before them; under audit each is
                                also evaluated in FP64 and compared */
+  uint64_t redo_rays;       /* two-pass march: rays marched again (more guard-band
+                               samples than a deferral slot holds, or no slot free) */
+  uint64_t deferred_rays;   /* two-pass march: rays whose guard-band samples were
+                               re-evaluated in FP64 after the march */
+  int32_t two_pass;         /* 1 when the guarded simple march ran in two passes: the
+                               march without FP64, guard-band samples re-evaluated after */
+  int32_t reserved1;
 } xrp_render_stats;
 
 int xrp_abi_version(void);

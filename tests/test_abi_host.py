@@ -264,3 +264,37 @@ def test_tanh_shear_scene_rules():
             X.Scene.from_dict(obj, bad)
         with pytest.raises(ValueError):
             O.build_deformation(bad)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror of a struct in include/xrp.h has the header's
+    size and the header's offset for each field (a C program built with gcc
+    prints them)."""
+    import subprocess
+    from xray_projection_render_amd import _lib as XL
+    pairs = {"xrp_render_params": XL.RenderParams, "xrp_render_stats": XL.RenderStats,
+             "xrp_view_angle": XL.ViewAngle, "xrp_camera": XL.Camera,
+             "xrp_batch_item": XL.BatchItem, "xrp_tile_shape": XL.TileShape,
+             "xrp_batch_stats": XL.BatchStats, "xrp_surrogate_config": XL.SurrogateConfig,
+             "xrp_train_config": XL.TrainConfig, "xrp_train_stats": XL.TrainStats,
+             "xrp_surrogate_stats": XL.SurrogateStats}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "xrp.h"', "int main(void) {"]
+    for cname, cls in pairs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                               text=True).stdout.splitlines():
+        cname, fname, val = line.split()
+        got[(cname, fname)] = int(val)
+    for cname, cls in pairs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

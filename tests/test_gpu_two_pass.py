@@ -1,0 +1,232 @@
+"""The two-pass guarded simple march (march.hip march_try_kernel,
+march_resolve_kernel, march_redo_kernel).  The first pass marches every ray
+without the FP64 evaluator and defers its guard-band samples; the resolve
+pass evaluates those in FP64 and adds their terms; rays with more of them
+than a ray may defer are marched again with FP64 re-evaluation.
+
+Every pixel must have the bits of the one-pass march (XRP_TWO_PASS=0, read
+per render), the stats must count the same samples, and the images stay
+within the oracle's 1e-5 bar.  The library takes two passes for jobs of
+2^21 rays and up; XRP_TWO_PASS=1 takes them for the small jobs here too.
+The parity suite renders under audit, which takes the one-pass march, so
+these renders are made without it."""
+import numpy as np
+import pytest
+
+import xray_projection_render_amd as X
+from xray_projection_render_amd import workloads as W
+from helpers import load_scene_file, oracle_render, oracle_scene, xrp_scene
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+ANGLES = [(0.0, 90.0), (33.0, 70.0), (90.0, 90.0), (200.0, 120.0)]
+
+
+def _file_case(name, deformation=None):
+    obj, numbers, _ = load_scene_file(name)
+    dfm = load_scene_file(deformation)[0] if deformation else None
+    return obj, dfm, numbers
+
+
+def _cases():
+    out = []
+    for name, dfm in [("cube_w_hole.yaml", None), ("box_w_pped.yaml", None),
+                      ("balls.json", None), ("gyroid_example.yaml", None),
+                      ("pillar_array.json", None), ("lattice.yaml", None),
+                      ("cube_w_hole.yaml", "deformation_linear.yaml"),
+                      ("box_w_pped.yaml", "deformation_sigmoid.yaml")]:
+        out.append(pytest.param("file", (name, dfm), id=f"{name}+{dfm}"))
+    out.append(pytest.param("workload", "csg", id="csg"))
+    out.append(pytest.param("workload", "lattice", id="lattice"))
+    out.append(pytest.param("workload", "gyroid", id="gyroid_tanh"))
+    for seed in (0, 1, 2, 5):
+        out.append(pytest.param("random", seed, id=f"random{seed}"))
+    return out
+
+
+def _setup(kind, arg):
+    """(object, deformation, numbers, angles, resolution, ds)."""
+    if kind == "file":
+        obj, dfm, numbers = _file_case(*arg)
+        sc = xrp_scene(obj, dfm, numbers=numbers)
+        ds = sc.auto_ds()
+        sc.close()
+        return obj, dfm, numbers, ANGLES, 64, ds
+    if kind == "workload":
+        w = {"csg": lambda: W.csg(resolution=96, n_views=3),
+             "lattice": lambda: W.lattice(resolution=64, n_views=2),
+             "gyroid": lambda: W.gyroid_lattice(resolution=64, n_views=2, ds=0.002,
+                                                shear=True)}[arg]()
+        return w.obj, w.deformation, "yaml", w.angles, w.resolution, w.ds
+    obj, _, _ = W.random_scene(arg, max_prims=20)
+    sc = xrp_scene(obj)
+    ds = sc.auto_ds()
+    sc.close()
+    return obj, None, "yaml", ANGLES, 64, ds
+
+
+@pytest.mark.parametrize("kind,arg", _cases())
+def test_two_pass_same_bits_counts_and_parity(ctx, monkeypatch, kind, arg):
+    obj, dfm, numbers, angles, res, ds = _setup(kind, arg)
+    sc = xrp_scene(obj, dfm, numbers=numbers)
+    monkeypatch.setenv("XRP_TWO_PASS", "1")
+    a, sa = X.render(ctx, sc, angles, res, ds, integrator="simple", precision="guarded")
+    monkeypatch.setenv("XRP_TWO_PASS", "0")
+    b, sb = X.render(ctx, sc, angles, res, ds, integrator="simple", precision="guarded")
+    sc.close()
+    print(f"{kind} {arg}: block {sa['block']} two_pass {sa['two_pass']} deferred_rays "
+          f"{sa['deferred_rays']} redo_rays {sa['redo_rays']} of {sa['rays']}, reevaluated "
+          f"{sa['reevaluated']}", flush=True)
+    assert sb["two_pass"] == 0 and sb["redo_rays"] == 0 and sb["deferred_rays"] == 0
+    assert sa["two_pass"] == (1 if sa["block"] in (1, 2) else 0)
+    assert np.array_equal(a, b)
+    assert sa["samples"] == sb["samples"] and sa["rays"] == sb["rays"]
+    fixed = kind != "random"
+    if fixed:
+        assert sa["skipped_samples"] == sb["skipped_samples"]
+        assert sa["reevaluated"] == sb["reevaluated"]
+    else:
+        # a root collection of 8 children or more is culled per wave, and
+        # both the guard bands tested and the safe radius cover only the
+        # wave's children: the marched-again rays share waves with other
+        # rays, so they may re-evaluate or settle a few samples more or
+        # fewer (the densities, and so the bits, are the same)
+        assert abs(sa["skipped_samples"] - sb["skipped_samples"]) <= 1e-3 * sb["samples"]
+        assert abs(sa["reevaluated"] - sb["reevaluated"]) <= 1e-2 * sb["reevaluated"] + 8
+    # every ray with a guard-band sample was resolved or marched again
+    if sa["two_pass"] and fixed:
+        moved = sa["redo_rays"] + sa["deferred_rays"]
+        assert (sa["reevaluated"] > 0) == (moved > 0)
+        assert moved <= sa["reevaluated"]
+    osc = oracle_scene(obj, dfm, numbers=numbers)
+    ref = oracle_render(osc, angles, res, ds, integrator="simple")
+    assert float(np.max(np.abs(a.astype(np.float64) - ref))) <= TOL
+
+
+@pytest.mark.parametrize("res,n_views,per_launch", [(2048, 9, "4"), (2896, 3, "1"),
+                                                    (2048, 5, "")])
+def test_two_pass_split_launches(ctx, monkeypatch, res, n_views, per_launch):
+    """Large images, in launches of a few views (XRP_LAUNCH_VIEWS, read per
+    render; a launch otherwise covers up to 2^31 rays): the same bits as
+    one pass."""
+    w = W.csg(resolution=res, n_views=n_views)
+    sc = xrp_scene(w.obj)
+    monkeypatch.setenv("XRP_TWO_PASS", "1")
+    if per_launch:
+        monkeypatch.setenv("XRP_LAUNCH_VIEWS", per_launch)
+    a, sa = X.render(ctx, sc, w.angles, res, w.ds, integrator="simple")
+    monkeypatch.delenv("XRP_LAUNCH_VIEWS", raising=False)
+    monkeypatch.setenv("XRP_TWO_PASS", "0")
+    b, sb = X.render(ctx, sc, w.angles, res, w.ds, integrator="simple")
+    sc.close()
+    print(f"{res}^2 x {n_views}: deferred_rays {sa['deferred_rays']} redo_rays "
+          f"{sa['redo_rays']}", flush=True)
+    assert sa["two_pass"] == 1 and sa["deferred_rays"] > 0
+    assert np.array_equal(a, b)
+    assert sa["samples"] == sb["samples"] and sa["reevaluated"] == sb["reevaluated"]
+
+
+def test_two_pass_gray8_output(ctx, monkeypatch):
+    w = W.csg(resolution=80, n_views=3)
+    sc = xrp_scene(w.obj)
+    monkeypatch.setenv("XRP_TWO_PASS", "1")
+    a, sa = X.render(ctx, sc, w.angles, w.resolution, w.ds, output="gray8_alpha8",
+                     transparency=True)
+    monkeypatch.setenv("XRP_TWO_PASS", "0")
+    b, _ = X.render(ctx, sc, w.angles, w.resolution, w.ds, output="gray8_alpha8",
+                    transparency=True)
+    sc.close()
+    assert sa["two_pass"] == 1
+    assert np.array_equal(a, b)
+
+
+def test_two_pass_by_job_size(ctx, monkeypatch):
+    """Without the switch, jobs under 2^21 rays take one pass and larger
+    ones two."""
+    monkeypatch.delenv("XRP_TWO_PASS", raising=False)
+    w = W.csg(resolution=1024, n_views=2)
+    sc = xrp_scene(w.obj)
+    _, st = X.render(ctx, sc, w.angles[:1], 1024, w.ds)
+    assert st["two_pass"] == 0
+    _, st = X.render(ctx, sc, w.angles, 1024, w.ds)
+    assert st["two_pass"] == 1
+    sc.close()
+
+
+def test_two_pass_only_where_it_applies(ctx, monkeypatch):
+    """Audit, the hierarchical integrator, FP64 and 4-sample blocks (24
+    nodes and up) keep the one-pass kernels."""
+    monkeypatch.setenv("XRP_TWO_PASS", "1")
+    w = W.csg(resolution=48, n_views=1)
+    sc = xrp_scene(w.obj)
+    _, st = X.render(ctx, sc, w.angles, 48, w.ds, audit=True)
+    assert st["two_pass"] == 0
+    _, st = X.render(ctx, sc, w.angles, 48, 10 * w.ds, integrator="hierarchical")
+    assert st["two_pass"] == 0
+    _, st = X.render(ctx, sc, w.angles, 48, w.ds, precision="fp64")
+    assert st["two_pass"] == 0
+    sc.close()
+    obj, _, _ = W.random_scene(3, max_prims=200)
+    sc = xrp_scene(obj)
+    _, st = X.render(ctx, sc, w.angles, 48, sc.auto_ds())
+    assert st["block"] == 4 and st["two_pass"] == 0
+    sc.close()
+
+
+@pytest.mark.parametrize("most", ["0", "1"])
+def test_two_pass_marches_again_what_does_not_fit(ctx, monkeypatch, most):
+    """With a ray's slot cut to hold at most 0 or 1 guard-band samples
+    (XRP_DEFER_MAX, read per render), rays with more stop in the first pass
+    and are marched again by the redo pass: the same bits and counts as
+    one pass."""
+    w = W.gyroid_lattice(resolution=64, n_views=2, ds=0.002, shear=True)
+    sc = xrp_scene(w.obj, w.deformation)
+    monkeypatch.setenv("XRP_TWO_PASS", "1")
+    monkeypatch.setenv("XRP_DEFER_MAX", most)
+    a, sa = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple")
+    monkeypatch.delenv("XRP_DEFER_MAX")
+    monkeypatch.setenv("XRP_TWO_PASS", "0")
+    b, sb = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple")
+    sc.close()
+    print(f"at most {most}: deferred_rays {sa['deferred_rays']} redo_rays {sa['redo_rays']}",
+          flush=True)
+    assert sa["two_pass"] == 1 and sa["redo_rays"] > 0
+    if most == "0":
+        assert sa["deferred_rays"] == 0
+    else:
+        assert sa["deferred_rays"] > 0
+    assert np.array_equal(a, b)
+    for key in ("samples", "reevaluated", "skipped_samples"):
+        assert sa[key] == sb[key], key
+
+
+@pytest.mark.parametrize("defer_cap,redo_cap,expect_two_pass", [("3", "100000", 1),
+                                                                ("3", "2", 0)])
+def test_two_pass_lists_full(ctx, monkeypatch, defer_cap, redo_cap, expect_two_pass):
+    """Lists shortened (XRP_DEFER_CAP, XRP_REDO_CAP, read per render): rays
+    finding the deferral list full are marched again; when the redo list
+    is full too the job is rendered again in one pass, and so are the
+    scene's later renders.  Either way the same bits as one pass."""
+    w = W.gyroid_lattice(resolution=64, n_views=2, ds=0.002, shear=True)
+    sc = xrp_scene(w.obj, w.deformation)
+    monkeypatch.setenv("XRP_TWO_PASS", "1")
+    monkeypatch.setenv("XRP_DEFER_CAP", defer_cap)
+    monkeypatch.setenv("XRP_REDO_CAP", redo_cap)
+    a, sa = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple")
+    monkeypatch.delenv("XRP_DEFER_CAP")
+    monkeypatch.delenv("XRP_REDO_CAP")
+    # a scene whose lists ran over stays in one pass
+    _, again = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple")
+    assert again["two_pass"] == expect_two_pass
+    monkeypatch.setenv("XRP_TWO_PASS", "0")
+    b, sb = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="simple")
+    sc.close()
+    print(f"caps {defer_cap}/{redo_cap}: two_pass {sa['two_pass']} deferred_rays "
+          f"{sa['deferred_rays']} redo_rays {sa['redo_rays']}", flush=True)
+    assert sa["two_pass"] == expect_two_pass
+    if expect_two_pass:
+        assert sa["deferred_rays"] <= 3 and sa["redo_rays"] > 0
+    assert np.array_equal(a, b)
+    for key in ("samples", "reevaluated", "skipped_samples"):
+        assert sa[key] == sb[key], key

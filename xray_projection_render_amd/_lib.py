@@ -66,7 +66,7 @@ class _OutStruct(ctypes.Structure):
 
     def as_dict(self) -> Dict:
         return {k: getattr(self, k) for k, _ in self._fields_
-                if k not in ("reserved", "reserved0", "struct_size")}
+                if k not in ("reserved", "reserved0", "reserved1", "struct_size")}
 
 
 class RenderStats(_OutStruct):
@@ -78,7 +78,9 @@ class RenderStats(_OutStruct):
         ("position_error", ctypes.c_double), ("precision_used", ctypes.c_int32),
         ("block", ctypes.c_int32), ("staged_chunks", ctypes.c_uint64),
         ("staged_voxels", ctypes.c_uint64), ("fallback_chunks", ctypes.c_uint64),
-        ("skipped_samples", ctypes.c_uint64),
+        ("skipped_samples", ctypes.c_uint64), ("redo_rays", ctypes.c_uint64),
+        ("deferred_rays", ctypes.c_uint64), ("two_pass", ctypes.c_int32),
+        ("reserved1", ctypes.c_int32),
     ]
 
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -137,11 +138,17 @@ struct xrp_context {
   xrp_scene* vol_scene = nullptr;
   // the fixed scratch region of xrp_render_device_noalloc on this device
   void* scratch = nullptr;
+  // ray list of the two-pass guarded march (march.hpp redo_list)
+  DevBuf redo;
 };
 
 struct xrp_scene {
   std::unique_ptr<xrp::Scene> s;
   std::map<xrp_context*, std::vector<void*>> uploads;  // voxel grids per context
+  // a two-pass render of this scene found its lists too short: the scene's
+  // guard-band samples are too many for the two passes to pay, so its
+  // renders stay in one pass
+  std::atomic<bool> one_pass{false};
 };
 
 namespace xrp {
@@ -434,6 +441,18 @@ bool reach_enabled() {
   return on;
 }
 
+// Two-pass guarded simple march (XRP_TWO_PASS=0 turns it off, for
+// comparison; 1 takes it for small jobs too, for tests; read per render;
+// the image is the same either way).
+bool two_pass_enabled() {
+  const char* e = std::getenv("XRP_TWO_PASS");
+  return !(e && std::strcmp(e, "0") == 0);
+}
+bool two_pass_forced() {
+  const char* e = std::getenv("XRP_TWO_PASS");
+  return e && std::strcmp(e, "1") == 0;
+}
+
 // Per-wave culling of a root collection's children (XRP_CULL=0 turns it
 // off, for comparison; the image is the same either way).
 bool cull_enabled() {
@@ -723,7 +742,7 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
   // second-order reach skip far enough that the second sample of a block is
   // mostly wasted: 1 is 6 % faster there (6^3 sheared gyroid cells, 256 x
   // 1024^2: 90.1 against 95.9 ms; 512^2 gyroid lattice 1.135 against 1.212).
-  mp.block = mp.vel ? 1 : s.nodes.size() >= 24 ? 4 : 2;
+  mp.block = s.nodes.size() >= 24 ? 4 : mp.vel ? 1 : 2;
   if (const char* e = std::getenv("XRP_BLOCK")) {  // for comparison: 1, 2 or 4
     const int b = std::atoi(e);
     if (b == 1 || b == 2 || b == 4) mp.block = b;
@@ -819,7 +838,8 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
 // region instead of the context's buffers, and a voxel grid must already be
 // resident on the context.
 int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
-                void* out_dev, xrp_render_stats* st, void* fixed = nullptr) {
+                void* out_dev, xrp_render_stats* st, void* fixed = nullptr,
+                bool one_pass = false) {
   using namespace xrp;
   Scene& s = *sc->s;
   const int nv = (int)job.views.size();
@@ -832,6 +852,23 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   const bool vox_kernel = pl.vox_kernel;
   const int layout = pl.layout;
   const double delta = pl.delta;
+  // Guarded simple march at block 1 or 2 in two passes (march.hip
+  // march_try_kernel), for jobs of 2^21 rays and up (below, the two extra
+  // launches cost more than they save: CSG 8 x 256^2 0.107 against 0.097
+  // ms).  A launch covers at most 2^31 rays, which the lists name by 32-bit
+  // index.  Not under audit, nor without allocation.
+  const int kMaxGridY = 65535;
+  const bool two_pass = !fixed && !one_pass && !sc->one_pass &&
+                        precision == XRP_PRECISION_GUARDED && !vox_kernel &&
+                        job.integrator == XRP_INTEGRATE_SIMPLE && !job.audit && !pl.mp.queue &&
+                        (pl.mp.block == 1 || pl.mp.block == 2) &&
+                        ((size_t)nv * res * res >= ((size_t)1 << 21) || two_pass_forced()) &&
+                        two_pass_enabled();
+  int vpc = two_pass ? (int)std::min<size_t>(kMaxGridY, ((size_t)1 << 31) / ((size_t)res * res))
+                     : kMaxGridY;
+  if (const char* e = std::getenv("XRP_LAUNCH_VIEWS"))  // fewer views per launch, for tests
+    vpc = std::max(1, std::min(vpc, std::atoi(e)));
+  const int n_chunks = (nv + vpc - 1) / vpc;
 
   // Sample table (device copy cached per context).
   const double smin = job.R - kCubeHalfDiagonal, smax = job.R + kCubeHalfDiagonal;
@@ -860,9 +897,13 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   const size_t b_stats = ST_COUNT * sizeof(unsigned long long);
   const size_t o_defs = align(b_nodes), o_views = o_defs + align(std::max<size_t>(b_defs, 1));
   const size_t o_stats = o_views + align(b_views);
+  // zeroed list counters of the two-pass march, one per launch
+  const size_t o_ctr = o_stats + align(b_stats);
+  const size_t b_ctr = two_pass ? (size_t)n_chunks * 4 * sizeof(uint32_t) : 0;
   // the no-allocation form also carries the sample table
-  const size_t o_table = o_stats + align(b_stats);
-  const size_t total = fixed ? o_table + pl.table.size() * sizeof(double) : o_stats + b_stats;
+  const size_t o_table = o_ctr + align(b_ctr);
+  const size_t total = fixed ? o_table + pl.table.size() * sizeof(double)
+                             : two_pass ? o_ctr + b_ctr : o_stats + b_stats;
   std::vector<unsigned char> pageable;
   unsigned char *hb, *db;
   if (fixed) {
@@ -886,6 +927,20 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   if (b_defs) std::memcpy(hb + o_defs, s.ddefs.data(), b_defs);
   std::memcpy(hb + o_views, job.views.data(), b_views);
   std::memset(hb + o_stats, 0, b_stats);
+  if (b_ctr) std::memset(hb + o_ctr, 0, b_ctr);
+  // the lists, sized for the largest launch (XRP_DEFER_CAP and
+  // XRP_REDO_CAP shorten them, for tests: 1 and up)
+  TwoPassCaps caps = two_pass_caps(two_pass ? (size_t)std::min(vpc, nv) * res * res : 0);
+  if (const char* e = std::getenv("XRP_DEFER_CAP"))
+    caps.defer = (uint32_t)std::min<long>(caps.defer, std::max(1L, std::atol(e)));
+  if (const char* e = std::getenv("XRP_REDO_CAP"))
+    caps.redo = (uint32_t)std::min<long>(caps.redo, std::max(1L, std::atol(e)));
+  // guard-band samples a ray may defer (XRP_DEFER_MAX, fewer for tests: 0
+  // and up)
+  uint32_t defer_max = kDeferMax;
+  if (const char* e = std::getenv("XRP_DEFER_MAX"))
+    defer_max = (uint32_t)std::min<long>(kDeferMax, std::max(0L, std::atol(e)));
+  if (two_pass) HIPCHK(ctx->redo.ensure(two_pass_bytes(caps)));
   HIPCHK(hipMemcpyAsync(db, hb, total, hipMemcpyHostToDevice, ctx->stream));
   // pageable staging: wait for the copy, so that no early return below can
   // free the vector while the copy still reads it
@@ -900,10 +955,29 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   (void)res;
 
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  const int kMaxGridY = 65535;
-  for (int v0 = 0; v0 < nv; v0 += kMaxGridY) {
+  for (int v0 = 0, c = 0; v0 < nv; v0 += vpc, ++c) {
     mp.view_offset = v0;
-    mp.n_views = std::min(kMaxGridY, nv - v0);
+    mp.n_views = std::min(vpc, nv - v0);
+    if (two_pass) {
+      auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+      unsigned char* b = (unsigned char*)ctx->redo.p;
+      mp.redo_list = (uint32_t*)b;
+      b += al((size_t)caps.redo * 4);
+      mp.defer_T = (double*)b;
+      b += al((size_t)caps.defer * 8);
+      mp.defer_ray = (uint32_t*)b;
+      b += al((size_t)caps.defer * 4);
+      mp.defer_cnt = (uint32_t*)b;
+      b += al((size_t)caps.defer * 4);
+      mp.defer_kfirst = (uint32_t*)b;
+      b += al((size_t)caps.defer * 4);
+      mp.defer_k = (uint32_t*)b;
+      mp.redo_cap = caps.redo;
+      mp.defer_cap = caps.defer;
+      mp.k_cap = caps.k;
+      mp.defer_max = defer_max;
+      mp.counts = (uint32_t*)(db + o_ctr) + 4 * c;
+    }
     if (job.output == XRP_OUTPUT_GRAY8_ALPHA8)
       mp.out8 = (unsigned char*)out_dev + (size_t)v0 * res * res * 2;
     else
@@ -922,12 +996,27 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   // the staging copy above has completed in stream order: reuse its stats
-  // section for the read-back
+  // section (and the two-pass list counters after it) for the read-back
   unsigned long long* hstp = (unsigned long long*)(hb + o_stats);
-  HIPCHK(hipMemcpyAsync(hstp, mp.stats, b_stats, hipMemcpyDeviceToHost, ctx->stream));
+  const size_t b_back = two_pass ? o_ctr + b_ctr - o_stats : b_stats;
+  HIPCHK(hipMemcpyAsync(hstp, mp.stats, b_back, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   unsigned long long hst[ST_COUNT];
   std::memcpy(hst, hstp, sizeof hst);
+  uint64_t redo_rays = 0;
+  bool overflow = false;
+  for (int c = 0; two_pass && c < n_chunks; ++c) {
+    uint32_t n[4];
+    std::memcpy(n, hb + o_ctr + 4 * c * sizeof(uint32_t), sizeof n);
+    redo_rays += n[0];
+    overflow = overflow || n[3] != 0;
+  }
+  // a redo list too short for the rays it had to hold: the two-pass images
+  // lack those rays, so the job is rendered again in one pass
+  if (overflow) {
+    sc->one_pass = true;
+    return render_core(ctx, sc, job, out_dev, st, fixed, true);
+  }
   if (st) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -945,6 +1034,9 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
     o.kernel_ms = ms;
     o.position_error = delta;
     o.precision_used = precision;
+    o.two_pass = two_pass ? 1 : 0;
+    o.redo_rays = redo_rays;
+    o.deferred_rays = hst[ST_DEFERRED];
     o.block = precision == XRP_PRECISION_GUARDED && !vox_kernel &&
                       job.integrator == XRP_INTEGRATE_SIMPLE
                   ? pl.mp.block

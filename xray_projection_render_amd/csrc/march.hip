@@ -18,6 +18,7 @@
 //    is accumulated from exactly the oracle's per-sample densities.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "eval_device.hpp"
@@ -65,6 +66,152 @@ march_kernel(MarchParams p) {
   RayCounts rc;
   march_ray<PREC, INTEG, 1, B, PREC == PREC_GUARDED && INTEG == INT_SIMPLE && (B == 4 || Q)>(
       p, blockIdx.y, i, j, i < p.res && j < p.res, rc);
+  flush_counts(p.stats, rc);
+}
+
+// Two-pass guarded simple march, first pass: every ray marched without the
+// FP64 evaluator (march_T_guarded_inline<B, true>), one launch over all
+// views like the one-pass kernel.  A ray with no guard-band sample stores
+// its pixel; one with a few is appended, with its T and the deferred
+// sample indices, to the resolve pass's lists; one with more, or finding
+// the lists full, to p.redo_list for march_redo_kernel (one atomic per
+// wave and list).  Without the FP64 path in the loop no lane waits on
+// another's FP64 re-evaluation.
+template <int B>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
+march_try_kernel(MarchParams p) {
+  __shared__ uint32_t kbuf[64 * kDeferMax];
+  const int lane = threadIdx.x;
+  const int TI = p.tile_j32 ? 2 : march_tile_i(INT_SIMPLE);
+  const int TJ = p.tile_j32 ? 32 : march_tile_j(INT_SIMPLE);
+  const int tiles_j = (p.res + TJ - 1) / TJ;
+  const int ti = blockIdx.x / tiles_j, tj = blockIdx.x - ti * tiles_j;
+  const int i = ti * TI + lane / TJ;
+  const int j = tj * TJ + lane % TJ;
+  const int v = blockIdx.y;
+  const bool valid = i < p.res && j < p.res;
+  RayCounts rc;
+  const Cull cl = wave_cull<1>(p, v, i, j, valid);
+  TryState ts;
+  ts.ks = kbuf + lane * kDeferMax;
+  const double T = valid ? march_T_guarded_inline<B, true>(p, v, i, j, rc, cl, &ts) : 0.0;
+  const unsigned id = ((unsigned)v * (unsigned)p.res + (unsigned)i) * (unsigned)p.res +
+                      (unsigned)j;
+  if (valid && !ts.aborted && ts.cnt == 0) store_pixel(p, v, i, j, T);
+  // rays with deferred samples: entries and their sample indices, in lane
+  // order (an exclusive scan of the counts over the wave)
+  bool defer = valid && !ts.aborted && ts.cnt > 0;
+  const unsigned long long dm = __ballot(defer);
+  if (dm) {
+    unsigned incl = defer ? ts.cnt : 0u;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned t = (unsigned)__shfl_up((int)incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const unsigned total = (unsigned)__shfl((int)incl, 63, 64);
+    unsigned eb = 0, kb = 0;
+    if (lane == 0) {
+      eb = atomicAdd(p.counts + 1, (unsigned)__popcll(dm));
+      kb = atomicAdd(p.counts + 2, total);
+    }
+    eb = (unsigned)__shfl((int)eb, 0, 64);
+    kb = (unsigned)__shfl((int)kb, 0, 64);
+    const bool fits = eb + (unsigned)__popcll(dm) <= p.defer_cap && kb + total <= p.k_cap;
+    if (defer && fits) {
+      const unsigned e = eb + (unsigned)__popcll(dm & __lanemask_lt());
+      const unsigned k0 = kb + incl - ts.cnt;
+      p.defer_T[e] = T;
+      p.defer_ray[e] = id;
+      p.defer_cnt[e] = ts.cnt;
+      p.defer_kfirst[e] = k0;
+      for (unsigned c = 0; c < ts.cnt; ++c) p.defer_k[k0 + c] = ts.ks[c];
+    } else if (defer) {
+      // the lists are full: the wave's rays go to the redo list, and its
+      // entries below the cap are marked empty for the resolve pass
+      const unsigned e = eb + (unsigned)__popcll(dm & __lanemask_lt());
+      if (e < p.defer_cap) p.defer_cnt[e] = 0u;
+      ts.aborted = true;
+    }
+  }
+  if (valid && !ts.aborted) {
+    rc.samples += ts.samples;
+    rc.skipped += ts.skipped;
+  }
+  const unsigned long long m = __ballot(valid && ts.aborted);
+  if (m) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(p.counts, (unsigned)__popcll(m));
+    base = (unsigned)__shfl((int)base, 0, 64);
+    const unsigned e = base + (unsigned)__popcll(m & __lanemask_lt());
+    if (valid && ts.aborted) {
+      if (e < p.redo_cap)
+        p.redo_list[e] = id;
+      else
+        p.counts[3] = 1u;  // no room: the caller renders again in one pass
+    }
+  }
+  flush_counts(p.stats, rc);
+}
+
+// Resolve pass: one lane per entry of the first pass's lists (grid-stride
+// over min(counts[1], defer_cap) entries).  The deferred samples are
+// evaluated in FP64 and their terms summed in sample order into Ta, as the
+// one-pass march sums its re-evaluated terms; the pixel is T + Ta.  A wave
+// whose entries did not all fit wrote none of them (its rays are in the
+// redo list), and entries past the cap are not read.
+__global__ __launch_bounds__(64) void march_resolve_kernel(MarchParams p) {
+  const unsigned n = min(p.counts[1], p.defer_cap), nk = min(p.counts[2], p.k_cap);
+  const unsigned rr = (unsigned)p.res * (unsigned)p.res;
+  const double ds = p.ds, dm = p.dm;
+  unsigned long long reeval = 0, rays = 0;
+  for (unsigned e = blockIdx.x * 64u + threadIdx.x; e < n; e += gridDim.x * 64u) {
+    const unsigned cnt = p.defer_cnt[e], k0 = p.defer_kfirst[e];
+    if (cnt == 0u || k0 + cnt > nk) continue;
+    const unsigned id = p.defer_ray[e];
+    const unsigned v = id / rr, pix = id - v * rr;
+    const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
+    const Ray r = make_ray(p, p.views[p.view_offset + (int)v], i, j);
+    double Ta = 0.0;
+    for (unsigned c = 0; c < cnt; ++c) {
+      const double sk = p.s_table[p.defer_k[k0 + c]];
+      const double ev = density64(p, r.o[0] + r.d[0] * sk, r.o[1] + r.d[1] * sk,
+                                  r.o[2] + r.d[2] * sk);
+      Ta += (ev * dm) * ds;
+    }
+    store_pixel(p, (int)v, i, j, p.defer_T[e] + Ta);
+    reeval += cnt;
+    ++rays;
+  }
+  if (p.stats) {
+    const unsigned long long a = wave_sum(reeval), b = wave_sum(rays);
+    if (threadIdx.x == 0) {
+      if (a) atomicAdd(p.stats + ST_REEVAL, a);
+      if (b) atomicAdd(p.stats + ST_DEFERRED, b);
+    }
+  }
+}
+
+// Second pass: the rays the first pass listed, 64 per wave in list order
+// (rays of one first-pass wave stay together), marched again.  Every lane
+// here has FP64 re-evaluations, so they wait in the per-wave LDS queue and
+// are drained 64 at a time (march_T_guarded, 3 waves per SIMD) rather than
+// one lane at a time: the queued form gives each pixel exactly the value
+// the one-pass march computes.  A grid-stride loop over the listed rays.
+template <int B>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void
+march_redo_kernel(MarchParams p) {
+  const unsigned n = min(p.counts[0], p.redo_cap);
+  const unsigned rr = (unsigned)p.res * (unsigned)p.res;
+  RayCounts rc;
+  for (unsigned base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+    const unsigned e = base + threadIdx.x;
+    const bool valid = e < n;
+    const unsigned id = valid ? p.redo_list[e] : 0u;
+    const unsigned v = id / rr, pix = id - v * rr;
+    const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
+    march_ray<PREC_GUARDED, INT_SIMPLE, 1, B, true>(p, (int)v, i, j, valid, rc);
+  }
   flush_counts(p.stats, rc);
 }
 
@@ -241,7 +388,21 @@ int launch_march(const MarchParams& p, int precision, int integrator,
     if (integrator == INT_HIERARCHICAL)
       hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_HIERARCHICAL, kBlock>), grid,
                          block, 0, s, p);
-    else if (p.block == 2 && p.queue)
+    else if (p.redo_list && !p.audit && !p.queue && (p.block == 1 || p.block == 2)) {
+      // two passes and the resolve pass between them; the grids of the last
+      // two stride over however many entries the first appended
+      const dim3 grid3(std::min((p.defer_cap + 63u) / 64u, 8192u));
+      const dim3 grid2(std::min((p.redo_cap + 63u) / 64u, 1024u));
+      if (p.block == 1) {
+        hipLaunchKernelGGL(march_try_kernel<1>, grid, block, 0, s, p);
+        hipLaunchKernelGGL(march_resolve_kernel, grid3, block, 0, s, p);
+        hipLaunchKernelGGL(march_redo_kernel<1>, grid2, block, 0, s, p);
+      } else {
+        hipLaunchKernelGGL(march_try_kernel<2>, grid, block, 0, s, p);
+        hipLaunchKernelGGL(march_resolve_kernel, grid3, block, 0, s, p);
+        hipLaunchKernelGGL(march_redo_kernel<2>, grid2, block, 0, s, p);
+      }
+    } else if (p.block == 2 && p.queue)
       hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_SIMPLE, 2, true>), grid, block,
                          0, s, p);
     else if (p.block == 2)

@@ -26,7 +26,9 @@ enum StatSlot {
   ST_STAGED_VOXELS = 5,   //   voxels copied into LDS
   ST_FALLBACK_CHUNKS = 6, //   chunks too large for LDS (gathered)
   ST_SKIPPED = 7,      // samples settled by a safe radius without evaluation
-  ST_COUNT = 8
+  ST_DEFERRED = 8,     // two-pass march: rays whose guard-band samples were
+                       // resolved by march_resolve_kernel
+  ST_COUNT = 9
 };
 
 struct MarchParams {
@@ -87,7 +89,58 @@ struct MarchParams {
   // Per-scene march kernel: rays of a wave as 2 x 32 pixels (32 along the
   // image's contiguous axis j) instead of 32 x 2.
   int32_t tile_j32;
+  // Guarded simple march at block 1 or 2 in two passes (march.hip) when
+  // redo_list is set.  The first pass (march_try_kernel) marches every ray
+  // without the FP64 evaluator and defers its guard-band samples (at most
+  // defer_max, <= kDeferMax; their indices wait in LDS).  At the end of a
+  // wave its rays with deferred samples are appended (one atomic per wave)
+  // as entries e < defer_cap: the ray (v * res^2 + i * res + j, v counted
+  // from view_offset) to defer_ray[e], its T without those samples to
+  // defer_T[e], their number to defer_cnt[e] and the sample indices to
+  // defer_k from defer_kfirst[e] on (k_cap entries).  counts[1] and
+  // counts[2] count entries and indices taken.  The resolve pass
+  // (march_resolve_kernel) evaluates them in FP64, adds their terms in
+  // sample order and stores the pixel.  A ray with more than defer_max
+  // such samples, or finding the lists full, stops and is appended to
+  // redo_list (counts[0], redo_cap entries); march_redo_kernel marches
+  // those rays again.  A ray finding redo_list full sets counts[3]: the
+  // caller renders the launch again in one pass.  counts starts zeroed.
+  uint32_t* redo_list;
+  uint32_t* counts;
+  double* defer_T;
+  uint32_t* defer_ray;
+  uint32_t* defer_cnt;
+  uint32_t* defer_kfirst;
+  uint32_t* defer_k;
+  uint32_t redo_cap, defer_cap, k_cap;
+  uint32_t defer_max;
 };
+
+// Guard-band samples a ray may defer to the resolve pass (the first pass
+// keeps them in LDS: 8 KiB per wave).  A ray with more is marched again,
+// by one wave with few rays, which takes long: on the sheared gyroid
+// lattice 8 left 78 rays of 33.5 M to the redo pass (0.9 ms).
+constexpr uint32_t kDeferMax = 32;
+// Two-pass list sizes for a launch of `rays` rays: room for every ray (two
+// sample indices each) up to 2^22 rays; beyond, a ray entry per 8 rays, a
+// sample index per 4 and a redo entry per 64, but never less than for 2^22.
+struct TwoPassCaps {
+  uint32_t redo, defer, k;
+};
+inline TwoPassCaps two_pass_caps(size_t rays) {
+  const size_t small = rays < ((size_t)1 << 22) ? rays : ((size_t)1 << 22);
+  auto cap = [&](size_t all, size_t share) {
+    const size_t c = all > share ? all : share;
+    return (uint32_t)(c < 64 ? 64 : c);
+  };
+  return {cap(small, rays / 64), cap(small, rays / 8), cap(2 * small, rays / 4)};
+}
+// Bytes of the two-pass buffers for those sizes (each array 256-byte aligned).
+inline size_t two_pass_bytes(const TwoPassCaps& c) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  return al((size_t)c.redo * 4) + al((size_t)c.defer * 8) + 3 * al((size_t)c.defer * 4) +
+         al((size_t)c.k * 4);
+}
 
 // VOX_LDS: per-wave sub-volumes staged in LDS from the linear copy, with
 // the 4^3-brick copy as the gather fallback for oversized sub-volumes.

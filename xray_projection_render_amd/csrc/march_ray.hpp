@@ -136,9 +136,24 @@ __device__ __forceinline__ double march_T_simple64(const MarchParams& p, int v, 
 // the samples inside a guard band.  Those terms are summed, in sample order,
 // into Ta, the others into T, and the result is T + Ta: the same two sums,
 // and so the same bits, as the queued form below.
-template <int B>
+// State of one ray in the first pass of the two-pass march.
+struct TryState {
+  uint32_t* ks = nullptr;  // the lane's kDeferMax sample indices (LDS)
+  unsigned cnt = 0;        // guard-band samples deferred
+  unsigned samples = 0, skipped = 0;  // the ray's counts, when it was not stopped
+  bool aborted = false;
+};
+
+// TRY: the first pass of the two-pass march, without the FP64 evaluator
+// (MarchParams::redo_list).  Guard-band samples are deferred: their indices
+// go to ts->ks and the march goes on; the return value is T without their
+// terms (Ta = 0), which the resolve pass adds.  A ray with more than
+// p.defer_max of them stops and sets `aborted`; the counts of a ray that
+// ends go to ts, not rc.  Not for audit.
+template <int B, bool TRY = false>
 __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, int v, int i,
-                                                         int j, RayCounts& rc, const Cull& cl) {
+                                                         int j, RayCounts& rc, const Cull& cl,
+                                                         TryState* ts = nullptr) {
   // per-ray counts fit 32 bits (a ray has fewer than 2^31 samples)
   unsigned n_samples = 0, n_reeval = 0, n_miss = 0, n_skip = 0;
   const double ds = p.ds, dm = p.dm;
@@ -185,11 +200,26 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
       unsigned amb = 0;
       float dist = INFINITY;
       root_blk<B, true>(p.nodes, P, live, vv, amb, dist, cl);
+      if constexpr (TRY) {
+        if (amb) {
+          unsigned m = amb;
+          while (m) {
+            const int q = __builtin_ctz(m);
+            m &= m - 1u;
+            if (ts->cnt < p.defer_max) ts->ks[ts->cnt] = (uint32_t)(k + q);
+            ++ts->cnt;
+          }
+          if (ts->cnt > p.defer_max) {
+            ts->aborted = true;
+            return 0.0;
+          }
+        }
+      }
       const bool last_amb = (amb >> (B - 1)) & 1u;
       // FP64 re-evaluation (under audit of every sample; the FP32-clear
       // ones must agree) through one call site
-      unsigned redo = p.audit ? live : amb;
-      while (redo) {
+      unsigned redo = TRY ? 0u : p.audit ? live : amb;
+      while (!TRY && redo) {
         const int q = __builtin_ctz(redo);
         redo &= redo - 1u;
         double vq = vv[0];
@@ -220,7 +250,7 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
         if (f >= 1.0f && k <= k_hi) {
           const int m = min(f < 65536.0f ? (int)f : 65536, k_hi - k + 1);
           const double rho_m = vv[B - 1];
-          if (p.audit) {
+          if (!TRY && p.audit) {
             for (int jj = 0; jj < m; ++jj) {
               double s = p.s_table[k + jj];
               double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
@@ -240,10 +270,16 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
       }
     }
   }
-  rc.samples += n_samples;
-  rc.reeval += n_reeval;
-  rc.miss += n_miss;
-  rc.skipped += n_skip;
+  if constexpr (TRY) {
+    // counted by the caller once the ray is known not to be marched again
+    ts->samples = n_samples;
+    ts->skipped = n_skip;
+  } else {
+    rc.samples += n_samples;
+    rc.reeval += n_reeval;
+    rc.miss += n_miss;
+    rc.skipped += n_skip;
+  }
   return T + Ta;
 }
 
