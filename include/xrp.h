@@ -408,6 +408,12 @@ int xrp_diag_l2_bandwidth(xrp_context *ctx, double *gbps);
  * argument in [-0.5, 0.5], against FP64; the FP32 gyroid evaluators' guard
  * bands assume at most 1.3e-7. */
 int xrp_diag_sincos_error(xrp_context *ctx, double *max_sin, double *max_cos);
+/* Diagnostic, no counterpart in the reference: the largest absolute errors
+ * against FP64 of the FP32 deformations' hardware tanh (every FP32 argument
+ * in [-20, 20]), logistic function ([-100, 100]) and e^-x ([0, 110]); the
+ * deformations' position error bounds assume at most 3e-7, 2e-7, 2e-7. */
+int xrp_diag_deform_error(xrp_context *ctx, double *max_tanh, double *max_sigmoid,
+                          double *max_expneg);
 
 /* ---- reference JSON API (api.go:82-180) ----------------------------------- */
 /* Renders as the reference's RenderProjections does (PNG frames,

@@ -24,3 +24,17 @@ def test_hardware_sincos_error_within_guard_assumption(ctx):
     print(f"hardware sin/cos max abs error {es:.4e} / {ec:.4e}", flush=True)
     assert 0.0 < es <= 1.3e-7
     assert 0.0 < ec <= 1.3e-7
+
+
+def test_hardware_deformation_functions_within_guard_assumption(ctx):
+    """The FP32 deformations evaluate tanh, the logistic function and e^-x
+    through the hardware base-2 exponential and reciprocal, and their
+    position error bounds assume absolute errors of at most 3e-7, 2e-7 and
+    2e-7 (kHwTanhErr, kHwSigmoidErr, kHwExpNegErr); the library re-measures
+    them over every FP32 argument of their ranges against FP64."""
+    et, es, ee = X.diag_deform_error(ctx)
+    print(f"hardware tanh / logistic / exp(-x) max abs error {et:.4e} / {es:.4e} / {ee:.4e}",
+          flush=True)
+    assert 0.0 < et <= 3.0e-7
+    assert 0.0 < es <= 2.0e-7
+    assert 0.0 < ee <= 2.0e-7

@@ -214,6 +214,7 @@ def lib() -> ctypes.CDLL:
         "xrp_diag_mfma_bf16": (I, [P, FP, FP, FP]),
         "xrp_diag_l2_bandwidth": (I, [P, DP]),
         "xrp_diag_sincos_error": (I, [P, DP, DP]),
+        "xrp_diag_deform_error": (I, [P, DP, DP, DP]),
         "RenderProjections": (ctypes.POINTER(ctypes.c_char), [ctypes.c_char_p]),
         "FreeString": (None, [ctypes.POINTER(ctypes.c_char)]),
     }
@@ -649,6 +650,16 @@ def diag_sincos_error(ctx: Context) -> Tuple[float, float]:
     a, b = ctypes.c_double(), ctypes.c_double()
     check(lib().xrp_diag_sincos_error(ctx.ptr, ctypes.byref(a), ctypes.byref(b)))
     return float(a.value), float(b.value)
+
+
+def diag_deform_error(ctx: Context) -> Tuple[float, float, float]:
+    """Largest absolute errors of the deformations' hardware tanh, logistic
+    function and e^-x over every FP32 argument of their ranges
+    (xrp_diag_deform_error)."""
+    a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib().xrp_diag_deform_error(ctx.ptr, ctypes.byref(a), ctypes.byref(b),
+                                      ctypes.byref(c)))
+    return float(a.value), float(b.value), float(c.value)
 
 
 def diag_l2_bandwidth(ctx: Context) -> float:

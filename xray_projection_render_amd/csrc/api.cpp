@@ -2339,6 +2339,33 @@ int xrp_diag_sincos_error(xrp_context* ctx, double* max_sin, double* max_cos) {
   });
 }
 
+int xrp_diag_deform_error(xrp_context* ctx, double* max_tanh, double* max_sigmoid,
+                          double* max_expneg) {
+  return guarded([&]() -> int {
+    if (!ctx || !max_tanh || !max_sigmoid || !max_expneg)
+      return fail(XRP_ERR_NULL, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf out;
+    HIPCHK(out.ensure(3 * sizeof(unsigned long long)));
+    hipError_t he = hipMemsetAsync(out.p, 0, 3 * sizeof(unsigned long long), ctx->stream);
+    if (he == hipSuccess) {
+      int e = xrp::launch_deform_error((unsigned long long*)out.p, ctx->stream);
+      if (e) he = (hipError_t)e;
+    }
+    unsigned long long h[3] = {0, 0, 0};
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(h, out.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    out.release();
+    if (he != hipSuccess) return fail(XRP_ERR_DEVICE, hipGetErrorString(he));
+    std::memcpy(max_tanh, &h[0], sizeof(double));
+    std::memcpy(max_sigmoid, &h[1], sizeof(double));
+    std::memcpy(max_expneg, &h[2], sizeof(double));
+    return XRP_OK;
+  });
+}
+
 int xrp_diag_l2_bandwidth(xrp_context* ctx, double* gbps) {
   return guarded([&]() -> int {
     if (!ctx || !gbps) return fail(XRP_ERR_NULL, "NULL argument");

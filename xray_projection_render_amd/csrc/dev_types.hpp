@@ -40,6 +40,18 @@ constexpr int kMaxDeform = 8;
 // the guard bands carry this bound.
 constexpr double kHwSinCosErr = 1.3e-7;
 
+// The FP32 deformations evaluate tanh, the logistic function 1 / (1 + e^-t)
+// and e^-x (x >= 0) through the hardware base-2 exponential and reciprocal
+// (eval_device.hpp tanh_hw, sigmoid_hw, expneg_hw).  Their largest absolute
+// errors against FP64 over every FP32 argument (tanh on [-20, 20],
+// beyond which both round to +-1; the logistic function on [-100, 100];
+// e^-x on [0, 110]; outside, the results are the exact limits) are at most
+// these bounds on MI355X (xrp_diag_deform_error re-measures them); the
+// deformations' position error bounds carry them.
+constexpr double kHwTanhErr = 3.0e-7;
+constexpr double kHwSigmoidErr = 2.0e-7;
+constexpr double kHwExpNegErr = 2.0e-7;
+
 // 256-byte node.  f[] holds FP32 copies for the guarded march, d[] the
 // FP64 values the exact evaluator uses (same values the reference uses).
 // g[] holds the FP32 guard bands, recomputed on the host for every render
@@ -78,7 +90,7 @@ static_assert(sizeof(DevNode) == 256, "DevNode must stay 256 bytes");
 //  g[] also holds, per node kind, the safe-radius terms (eval_block.hpp):
 //          GYROID [3] 1/scale, [4] radius factor, [5] the deformation chain's
 //          Lipschitz bound, [6] the curvature bound M of the second-order
-//          reach (0: none), [7] the slope's error bound; TESS [2] 1 when the
+//          reach (0: none), [1] 1/M, [7] the slope's error bound; TESS [2] 1 when the
 //          content is seamless (a whole number of gyroid periods per cell).
 
 struct alignas(16) DevDeform {

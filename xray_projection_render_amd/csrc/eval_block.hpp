@@ -226,19 +226,23 @@ __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
 // same way inside.  The first root of c + b s - M s^2 / 2, written without
 // cancellation: (b + sqrt(b^2 + 2 M c)) / M for b > 0, else
 // 2c / (sqrt(b^2 + 2 M c) - b).
-__device__ __forceinline__ float quad_reach(float c, float b, float M) {
+// The division by M is a product with 1/M (n.g[1], rounded to nearest) and
+// the other one the hardware reciprocal (within an ulp): a few ulps in all,
+// which the final 1e-4 shrink covers.
+__device__ __forceinline__ float quad_reach(float c, float b, float M, float inv_M) {
   if (!(c > 0.0f)) return 0.0f;
   const float q = __builtin_amdgcn_sqrtf(fmaf(b, b, 2.0f * M * c));
-  return b > 0.0f ? (b + q) / M : 2.0f * c / (q - b);
+  return b > 0.0f ? (b + q) * inv_M : 2.0f * c * __builtin_amdgcn_rcpf(q - b);
 }
 __device__ __forceinline__ float gyroid_reach(const DevNode& n, float gg, float g1) {
-  const float th = n.f[4], band = n.g[0], M = n.g[6], e1 = n.g[7];
+  const float th = n.f[4], band = n.g[0], M = n.g[6], e1 = n.g[7], inv_M = n.g[1];
   const float ag = fabsf(gg);
   float s;
   if (ag >= th) {
-    s = quad_reach(ag - th - band, (gg < 0.0f ? -g1 : g1) - e1, M);
+    s = quad_reach(ag - th - band, (gg < 0.0f ? -g1 : g1) - e1, M, inv_M);
   } else {
-    s = fminf(quad_reach(th - band - gg, -g1 - e1, M), quad_reach(gg + th - band, g1 - e1, M));
+    s = fminf(quad_reach(th - band - gg, -g1 - e1, M, inv_M),
+              quad_reach(gg + th - band, g1 - e1, M, inv_M));
   }
   return s * (1.0f - 1e-4f);
 }

@@ -446,6 +446,25 @@ __device__ __forceinline__ GR root32(const DevNode* __restrict__ N, float x,
   return leaf32<0>(N, 0, x, y, z);
 }
 
+// tanh, the logistic function and e^-x from the hardware base-2
+// exponential (v_exp_f32) and reciprocal (v_rcp_f32), a handful of
+// instructions each where the library functions branch on the argument.
+// Absolute errors: kHwTanhErr, kHwSigmoidErr, kHwExpNegErr (dev_types.hpp).
+// tanh x = sign(x) (1 - 2 / (e^(2|x|) + 1)); past |x| = 44 the exponential
+// is infinite and the result exactly +-1.
+__device__ __forceinline__ float tanh_hw(float x) {
+  const float e = __builtin_amdgcn_exp2f(fabsf(x) * 2.88539008f);  // 2 log2(e)
+  return copysignf(fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f), x);
+}
+// 1 / (1 + e^-t)
+__device__ __forceinline__ float sigmoid_hw(float t) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t * -1.44269504f));
+}
+// e^-x for x >= 0
+__device__ __forceinline__ float expneg_hw(float x) {
+  return __builtin_amdgcn_exp2f(x * -1.44269504f);
+}
+
 __device__ __forceinline__ void deform32(const DevDeform* __restrict__ D,
                                          int nd, float& x, float& y,
                                          float& z) {
@@ -455,9 +474,9 @@ __device__ __forceinline__ void deform32(const DevDeform* __restrict__ D,
       case D_GAUSS: {
         float x0 = x - f.f[6], y0 = y - f.f[7], z0 = z - f.f[8];
         float r2 = fmaf(x0, x0, fmaf(y0, y0, z0 * z0));
-        float ddx = f.f[0] * expf(-r2 * f.f[3]);
-        float ddy = f.f[1] * expf(-r2 * f.f[4]);
-        float ddz = f.f[2] * expf(-r2 * f.f[5]);
+        float ddx = f.f[0] * expneg_hw(r2 * f.f[3]);
+        float ddy = f.f[1] * expneg_hw(r2 * f.f[4]);
+        float ddz = f.f[2] * expneg_hw(r2 * f.f[5]);
         x = x + ddx;
         y = y + ddy;
         z = z + ddz;
@@ -490,7 +509,7 @@ __device__ __forceinline__ void deform32(const DevDeform* __restrict__ D,
         break;
       case D_SIGMOID: {
         float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
-        q = q + f.f[0] / (1.0f + expf(-(q - f.f[1]) * f.f[2]));
+        q = q + f.f[0] * sigmoid_hw((q - f.f[1]) * f.f[2]);
         if (f.axis == 0)
           x = q;
         else if (f.axis == 1)
@@ -502,7 +521,7 @@ __device__ __forceinline__ void deform32(const DevDeform* __restrict__ D,
       case D_TANH: {
         float s = f.src == 0 ? x : (f.src == 1 ? y : z);
         float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
-        q = q + f.f[0] * tanhf(f.f[1] * s);
+        q = q + f.f[0] * tanh_hw(f.f[1] * s);
         if (f.axis == 0)
           x = q;
         else if (f.axis == 1)
@@ -531,7 +550,8 @@ __device__ __forceinline__ void deform32_jvp(const DevDeform* __restrict__ D, in
         float x0 = x - f.f[6], y0 = y - f.f[7], z0 = z - f.f[8];
         float r2 = fmaf(x0, x0, fmaf(y0, y0, z0 * z0));
         float rv = fmaf(x0, vx, fmaf(y0, vy, z0 * vz));
-        float ex = expf(-r2 * f.f[3]), ey = expf(-r2 * f.f[4]), ez = expf(-r2 * f.f[5]);
+        float ex = expneg_hw(r2 * f.f[3]), ey = expneg_hw(r2 * f.f[4]),
+              ez = expneg_hw(r2 * f.f[5]);
         float ddx = f.f[0] * ex, ddy = f.f[1] * ey, ddz = f.f[2] * ez;
         vx = fmaf(-2.0f * f.f[3] * ddx, rv, vx);
         vy = fmaf(-2.0f * f.f[4] * ddy, rv, vy);
@@ -582,10 +602,9 @@ __device__ __forceinline__ void deform32_jvp(const DevDeform* __restrict__ D, in
         // q += A sigma((q - c) / l): dq'/dq = 1 + (A / l) sigma (1 - sigma)
         // (the position rounds as deform32's)
         float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
-        float den = 1.0f + expf(-(q - f.f[1]) * f.f[2]);
-        float sg = 1.0f / den;
+        float sg = sigmoid_hw((q - f.f[1]) * f.f[2]);
         float gain = fmaf(f.f[0] * f.f[2] * sg, 1.0f - sg, 1.0f);
-        q = q + f.f[0] / den;
+        q = q + f.f[0] * sg;
         if (f.axis == 0) {
           x = q;
           vx *= gain;
@@ -603,7 +622,7 @@ __device__ __forceinline__ void deform32_jvp(const DevDeform* __restrict__ D, in
         float s = f.src == 0 ? x : (f.src == 1 ? y : z);
         float vs = f.src == 0 ? vx : (f.src == 1 ? vy : vz);
         float q = f.axis == 0 ? x : (f.axis == 1 ? y : z);
-        float th = tanhf(f.f[1] * s);
+        float th = tanh_hw(f.f[1] * s);
         float dv = f.f[0] * f.f[1] * fmaf(-th, th, 1.0f) * vs;
         q = q + f.f[0] * th;
         if (f.axis == 0) {

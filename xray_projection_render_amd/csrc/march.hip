@@ -356,6 +356,44 @@ int launch_sincos_error(unsigned* out, void* stream) {
   return (int)hipGetLastError();
 }
 
+// Largest absolute error against FP64 of tanh_hw (which = 0), sigmoid_hw
+// (1) or expneg_hw (2) over every FP32 argument of magnitude bits <= hi
+// (both signs for tanh and the logistic function, x >= 0 for e^-x), as
+// the bits of a non-negative double, maxed into *out.
+__global__ __launch_bounds__(256) void deform_error_kernel(int which, unsigned hi,
+                                                           unsigned long long* out) {
+  const unsigned stride = gridDim.x * blockDim.x;
+  double e = 0.0;
+  for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b <= hi; b += stride) {
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      if (which == 2 && sgn) continue;
+      const float t = __uint_as_float(b | (sgn ? 0x80000000u : 0u));
+      double d;
+      if (which == 0)
+        d = (double)tanh_hw(t) - tanh((double)t);
+      else if (which == 1)
+        d = (double)sigmoid_hw(t) - 1.0 / (1.0 + exp(-(double)t));
+      else
+        d = (double)expneg_hw(t) - exp(-(double)t);
+      e = fmax(e, fabs(d));
+    }
+    if (b > hi - stride) break;  // the next b would pass hi (or wrap)
+  }
+  atomicMax(out, (unsigned long long)__double_as_longlong(e));
+}
+
+int launch_deform_error(unsigned long long* out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(deform_error_kernel, dim3(8192), dim3(256), 0, s, 0,
+                     0x41A00000u /* 20.0f */, out);
+  hipLaunchKernelGGL(deform_error_kernel, dim3(8192), dim3(256), 0, s, 1,
+                     0x42C80000u /* 100.0f */, out + 1);
+  hipLaunchKernelGGL(deform_error_kernel, dim3(8192), dim3(256), 0, s, 2,
+                     0x42DC0000u /* 110.0f */, out + 2);
+  return (int)hipGetLastError();
+}
+
 int launch_volume(const MarchParams& p, int res, double* dens, unsigned long long* maxbits,
                   unsigned char* out, void* stream) {
   hipStream_t s = (hipStream_t)stream;

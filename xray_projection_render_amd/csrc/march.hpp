@@ -176,6 +176,11 @@ int launch_l2_probe(const void* buf, int workgroups, int iters, unsigned* out, v
 // revolutions: out[0], out[1] (two uints, zeroed by the caller) receive the
 // bits of the largest absolute errors of sin and cos.
 int launch_sincos_error(unsigned* out, void* stream);
+// Hardware tanh, logistic function and e^-x (eval_device.hpp) against FP64
+// over every FP32 argument of their ranges (dev_types.hpp kHwTanhErr ...):
+// out[0..2] (zeroed by the caller) receive the bits of the largest absolute
+// errors, as non-negative doubles.
+int launch_deform_error(unsigned long long* out, void* stream);
 // Exact march of a list: with vals, items are (view, i, j) pixels and
 // vals[k] receives the transmittance; without, items are (view, ti, tj)
 // 16x16 tiles rendered into p.out.

@@ -1128,19 +1128,25 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
         break;
       }
       case D_SIGMOID: {
+        // Lipschitz A / (4 ls); the argument (q - c) / ls rounded (at most
+        // u arg, through a slope of at most 1/4, kept whole here), the
+        // hardware logistic function's kHwSigmoidErr, the product with A
+        // and the add.
         double A = std::fabs(f.p[0]), c = std::fabs(f.p[1]), ls = std::fabs(f.p[2]);
         double arg = (B + c) / ls;
-        delta = delta * (1 + A / (4 * ls)) + A * (u * arg + 8 * u) + 4 * u * (B + A);
+        delta = delta * (1 + A / (4 * ls)) + A * (u * arg + kHwSigmoidErr + 4 * u) +
+                4 * u * (B + A);
         B = B + A;
         lip_chain *= 1 + A / (4 * ls);  // max slope of A / (1 + e^(-x/ls))
         h = A / (ls * ls) * 0.0963;     // max |sigma''| = 1 / (6 sqrt 3)
         break;
       }
       case D_TANH: {
-        // Lipschitz |A| k; tanhf within a few ulp; FP32 rounding of the
-        // amplitude, the rate product and the add.
+        // Lipschitz |A| k; the hardware tanh's kHwTanhErr; FP32 rounding of
+        // the amplitude, the rate product and the add.
         double A = std::fabs(f.p[0]), k = std::fabs(f.p[1]);
-        delta = delta * (1 + A * k) + A * k * 2 * u * B + 8 * u * A + 2 * u * (B + A);
+        delta = delta * (1 + A * k) + A * k * 2 * u * B + A * (kHwTanhErr + 4 * u) +
+                2 * u * (B + A);
         B = B + A;
         // Euclidean: the Jacobian is I + a e_axis e_source^T with |a| <= A k
         // (axis != source), a shear whose largest singular value is
@@ -1157,7 +1163,10 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
           lip = std::max(lip, A * 1.06 / sg);
           amax = std::max(amax, A);
         }
-        delta = delta * (1 + lip) + 12 * u * amax + 4 * u * (B + amax);
+        // the hardware e^-x's kHwExpNegErr, the exponent r^2 / (2 s^2)
+        // rounded (relative 4u, through x e^-x <= 1/e: 1.5u), the product
+        // with A and the add
+        delta = delta * (1 + lip) + amax * (kHwExpNegErr + 4 * u) + 4 * u * (B + amax);
         B = B + amax;
         // |grad A e^(-r^2/2s^2)| <= |A| e^(-1/2) / s per component, three
         // components: sqrt(3) * 0.6066 < 1.06
@@ -1296,6 +1305,7 @@ double compute_guards(Scene& s, double world_bound, double step, double* lipschi
           n.g[3] = (float)inv;
           n.g[5] = (float)lip_chain;
           n.g[6] = std::isfinite(M) && M < 1e30 ? (float)M : 0.0f;
+          n.g[1] = n.g[6] > 0.0f ? (float)(1.0 / (double)n.g[6]) : 0.0f;
           n.g[7] = (float)(2 * lip_chain * inv * (eg + 128 * u));
         }
         break;
