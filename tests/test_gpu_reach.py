@@ -128,3 +128,45 @@ def test_reach_settles_more_on_the_sheared_lattice(ctx, no_reach):
     print(f"settled {st['skipped_samples']} vs {before} of {st['samples']}", flush=True)
     assert st["skipped_samples"] > 1.05 * before
     sc.close()
+
+
+def _seamed_cases():
+    """Tessellations whose cell content is not continuous across the cell
+    faces: gyroids of 1.5 and 0.7 periods per cell, and a sphere set off the
+    cell's centre so that the faces cut it."""
+    out = []
+    for periods in (1.5, 0.7):
+        w = W.gyroid_lattice(resolution=64, n_views=2, ds=0.002, shear=False)
+        g = w.obj["uc"]["objects"]["objects"][0]
+        g["scale"] = g["scale"] / periods
+        out.append((f"gyroid{periods}", w.obj))
+    cell = 0.25
+    out.append(("sphere_cut", {
+        "type": "tessellated_obj_coll",
+        "uc": {"objects": {"type": "object_collection", "objects": [
+            {"type": "sphere", "center": [0.06, -0.04, 0.02], "radius": 0.1, "rho": 1.0}]},
+               "xmin": -cell / 2, "xmax": cell / 2, "ymin": -cell / 2, "ymax": cell / 2,
+               "zmin": -cell / 2, "zmax": cell / 2},
+        "xmin": -0.5, "xmax": 0.5, "ymin": -0.5, "ymax": 0.5, "zmin": -0.5, "zmax": 0.5}))
+    return out
+
+
+@pytest.mark.parametrize("integ", ["simple", "hierarchical"])
+@pytest.mark.parametrize("dname", ["none", "tanh"])
+def test_seamed_tessellations_settle_exactly(ctx, integ, dname):
+    """Where the cell content jumps at the faces, a safe radius may cross a
+    face only when the neighbouring cell's density at the sample is the
+    sample's own: with audit every settled sample is compared in FP64."""
+    angles = [(20.0, 90.0), (133.0, 70.0)]
+    d = None if dname == "none" else W.TANH_SHEAR
+    for name, obj in _seamed_cases():
+        sc = xrp_scene(obj, d)
+        ds = 0.002 if integ == "simple" else 0.02
+        img, st = X.render(ctx, sc, angles, 64, ds, integrator=integ, audit=True)
+        print(f"{name} {dname} {integ}: skipped {st['skipped_samples']} of {st['samples']}, "
+              f"mismatches {st['audit_mismatch']}", flush=True)
+        assert st["audit_mismatch"] == 0, name
+        ref = oracle_render(oracle_scene(obj, d, numbers="json"), angles, 64, ds,
+                            integrator=integ)
+        assert float(np.max(np.abs(img.astype(np.float64) - ref))) <= TOL, name
+        sc.close()

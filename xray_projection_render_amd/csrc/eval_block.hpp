@@ -406,11 +406,32 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
         qy = fmaf(sh, sy, qy);
       else
         qz = fmaf(sh, sz, qz);
-      // the neighbour's content is this cell's translated: the path keeps
-      // its velocity there (zero when the march carries none)
-      float dn = 0.0f;
-      if constexpr (D + 1 <= kMaxDepth) dn = coll_dist(N, n.first, qx, qy, qz, p.vx, p.vy, p.vz);
-      dist = fminf(dist, fminf(f2, dn));
+      // Past the face the density is the neighbour's, whose content is
+      // this cell's translated: constant within its bound dn around this
+      // point (the path keeps its velocity there, zero when the march
+      // carries none), but equal to this sample's only where the two cells
+      // agree.  The content need not be continuous across faces (a
+      // gyroid of 1.5 periods per cell is not), so the neighbour's density
+      // at this point is evaluated; unless it is clear and equal, the
+      // radius stops at the face.
+      float cap = f1;
+      if constexpr (D + 1 <= kMaxDepth) {
+        // one walk of the content at the neighbour's point: its density,
+        // whether that is clear, and its bound
+        PBlock<1> nq;
+        nq.x[0] = qx;
+        nq.y[0] = qy;
+        nq.z[0] = qz;
+        nq.vx = p.vx;
+        nq.vy = p.vy;
+        nq.vz = p.vz;
+        double nv[1];
+        unsigned namb = 0;
+        float dn = INFINITY;
+        coll_blk<D + 1, 1, true>(N, n.first, nq, 1u, nv, namb, dn);
+        if (!namb && nv[0] == out[S - 1]) cap = fminf(f2, dn);
+      }
+      dist = fminf(dist, cap);
     }
   }
 }
@@ -526,6 +547,7 @@ __device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PB
   float unused = 0.0f;
   root_blk<S, false>(N, p, live, out, amb, unused, cl);
 }
+
 
 // With vel (wave-uniform) the last sample's velocity (p.vx..vz, the ray
 // direction on entry) is carried through the chain as well.
