@@ -345,6 +345,7 @@ def batch_cpu_baseline(ws, every: int = 4) -> dict:
 
 BF16_PEAK_TFLOPS = 2500.0
 SURROGATE_FLOPS_PER_SAMPLE = 2.0 * (32 * 64 + 64 * 64 + 64)  # MLP multiply-adds
+SURROGATE_GATHER_BYTES = 16 * 8 * 4  # hash-grid corners per sample x 2 BF16 features
 
 
 def bench_surrogate(args) -> int:
@@ -550,6 +551,7 @@ def c5_surrogate_leg(args, ctx, sc, w, exact, kw, plain_ms: float) -> dict:
     ds_sur, tol = 2 * w.ds, 1e-3
     out = {"config": f"hash grid 16 x 2^19 x 2, MLP 32-64-64-1 (BF16 MFMA), step {ds_sur:g} "
                      f"(midpoint), tolerance {tol:g}", "train": tr}
+    l2 = X.diag_l2_bandwidth(ctx)
     for mode, t in (("certified", -tol), ("sampled", tol)):
         sur.render_device(w.angles, res, w.ds, ds_sur, t, img.data_ptr(), **kw)  # warm-up
         torch.cuda.synchronize()
@@ -570,6 +572,19 @@ def c5_surrogate_leg(args, ctx, sc, w, exact, kw, plain_ms: float) -> dict:
             "check_pixels": st["check_pixels"], "network_samples": st["network_samples"],
             "exact_samples": st["exact_samples"], "network_ms": st["surrogate_ms"],
             "check_ms": st["check_ms"], "fallback_ms": st["fallback_ms"]}
+        # the network march against its two ceilings: BF16 MFMA for the MLP,
+        # and the hashed gathers (16 levels x 8 corners x 4 B per network
+        # sample) against the measured L2 read bandwidth
+        net_s = st["surrogate_ms"] / 1000.0
+        if net_s > 0:
+            tf = st["network_samples"] * SURROGATE_FLOPS_PER_SAMPLE / net_s / 1e12
+            gb = st["network_samples"] * SURROGATE_GATHER_BYTES / net_s / 1e9
+            out[mode]["roofline"] = {
+                "bound": "mfma", "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / BF16_PEAK_TFLOPS, "kernel": "sur_render_kernel",
+                "kernel_ms": st["surrogate_ms"],
+                "gathers": {"achieved": gb, "unit": "GB/s", "l2_peak_measured": l2,
+                            "frac": gb / l2 if l2 > 0 else None}}
         print(f"c5: surrogate {mode}: {ms:.0f} ms, max error {float(diff.max()):.2e}",
               file=sys.stderr, flush=True)
     sur.close()

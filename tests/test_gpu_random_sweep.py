@@ -58,3 +58,87 @@ def test_random_scene_sweep(ctx, seed):
         assert st["audit_mismatch"] == 0, (seed, integ)
         assert err <= TOL, (seed, integ)
     sc.close()
+
+
+_NONLINEAR = [
+    None,
+    {"type": "tanh_shear", "amplitude": 0.1, "rate": 8.0, "direction": "z", "source": "x"},
+    {"type": "sigmoid", "amplitude": 0.05, "center": 0.1, "lengthscale": 0.08,
+     "direction": "y"},
+    {"type": "gaussian", "amplitudes": [0.03, -0.02, 0.04], "sigmas": [0.2, 0.3, 0.25],
+     "centers": [0.1, 0.0, -0.1]},
+    {"type": "composed", "deformations": [
+        {"type": "rigid", "displacements": [0.01, -0.02, 0.0]},
+        {"type": "tanh_shear", "amplitude": 0.05, "rate": 6.0, "direction": "x",
+         "source": "y"}]},
+]
+
+
+def _random_tessellation(seed: int):
+    """A tessellation of a random unit cell: 1-5 primitives (spheres, boxes,
+    cylinders, gyroids of random period) placed so that the cell faces may
+    cut them, in cells of random size and offset, under one of the
+    nonlinear deformations."""
+    rng = np.random.default_rng(500 + seed)
+    cell = float(rng.uniform(0.12, 0.3))
+    off = rng.uniform(-0.3, 0.3, 3) * cell
+    lo = -cell / 2 + off
+    objs = []
+    for _ in range(int(rng.integers(1, 6))):
+        kind = ["sphere", "box", "cylinder", "gyroid"][int(rng.integers(0, 4))]
+        c = (lo + rng.uniform(0.0, 1.0, 3) * cell).tolist()
+        size = float(rng.uniform(0.15, 0.5)) * cell
+        rho = float(rng.choice([1.0, 0.5, -0.5]))
+        if kind == "sphere":
+            objs.append({"type": "sphere", "center": c, "radius": size, "rho": rho})
+        elif kind == "box":
+            objs.append({"type": "box", "center": c,
+                         "sides": [size, float(rng.uniform(0.2, 0.8)) * cell, size], "rho": rho})
+        elif kind == "cylinder":
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            objs.append({"type": "cylinder", "p0": (np.array(c) - size * u).tolist(),
+                         "p1": (np.array(c) + size * u).tolist(), "radius": 0.3 * size,
+                         "rho": rho})
+        else:
+            periods = float(rng.choice([1.0, 1.5, 0.7, 2.0]))
+            objs.append({"type": "gyroid", "center": c,
+                         "scale": cell / (2.0 * np.pi * periods),
+                         "thickness": float(rng.uniform(0.05, 0.2)), "rho": rho})
+    obj = {"type": "tessellated_obj_coll",
+           "uc": {"objects": {"type": "object_collection", "objects": objs},
+                  "xmin": float(lo[0]), "xmax": float(lo[0] + cell),
+                  "ymin": float(lo[1]), "ymax": float(lo[1] + cell),
+                  "zmin": float(lo[2]), "zmax": float(lo[2] + cell)},
+           "xmin": -0.45, "xmax": 0.45, "ymin": -0.45, "ymax": 0.4, "zmin": -0.4, "zmax": 0.45}
+    return obj, _NONLINEAR[seed % len(_NONLINEAR)]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 4, 6, 7, 8, 9, 10])  # 5: negative content only
+def test_random_tessellation_sweep(ctx, monkeypatch, seed):
+    """Random tessellations under nonlinear deformations: every settled or
+    FP32-decided sample audited in FP64 (both integrators), and the
+    two-pass march without audit, against the FP64 oracle at 1e-5."""
+    obj, dfm = _random_tessellation(seed)
+    sc = xrp_scene(obj, dfm)
+    ds = 0.003
+    rng = np.random.default_rng(seed)
+    angles = [(float(rng.uniform(0, 360)), float(rng.uniform(20, 160))) for _ in range(2)]
+    osc = oracle_scene(obj, dfm)
+    for integ in ("simple", "hierarchical"):
+        d = ds if integ == "simple" else 10 * ds
+        img, st = X.render(ctx, sc, angles, 40, d, integrator=integ, audit=True)
+        ref = oracle_render(osc, angles, 40, d, integrator=integ)
+        err = float(np.max(np.abs(img.astype(np.float64) - ref)))
+        print(f"seed {seed} {integ}: {len(obj['uc']['objects']['objects'])} children, "
+              f"max_abs {err:.2e}, audit {st['audit_mismatch']}, skipped "
+              f"{st['skipped_samples']} of {st['samples']}", flush=True)
+        assert st["audit_mismatch"] == 0, (seed, integ)
+        assert err <= TOL, (seed, integ)
+        if integ == "simple":
+            monkeypatch.setenv("XRP_TWO_PASS", "1")
+            img2, st2 = X.render(ctx, sc, angles, 40, d, integrator=integ)
+            monkeypatch.delenv("XRP_TWO_PASS")
+            assert st2["two_pass"] == 1
+            assert np.array_equal(img2, img), seed
+    sc.close()
