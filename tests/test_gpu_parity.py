@@ -18,6 +18,9 @@ from helpers import load_scene_file, oracle_render, oracle_scene, xrp_scene, SCE
 
 pytestmark = pytest.mark.gpu
 
+# the child processes below import the package from the repository root
+_REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
 TOL = 1e-5
 
 
@@ -176,7 +179,7 @@ def test_safe_radius_settling_changes_no_bits(ctx, tmp_path):
     path = tmp_path / "noskip.npz"
     env = dict(os.environ, XRP_SKIP="0")
     subprocess.run([_sys.executable, "-c", _NO_SKIP_SCRIPT, str(path)], env=env, check=True,
-                   timeout=300)
+                   timeout=300, cwd=_REPO)
     ref = np.load(path)
     for name in ("csg", "lattice", "gyroid"):
         if name == "csg":
@@ -229,7 +232,7 @@ def test_child_culling_changes_no_bits(ctx, tmp_path):
     path = tmp_path / "nocull.npz"
     env = dict(os.environ, XRP_CULL="0")
     subprocess.run([_sys.executable, "-c", _NO_CULL_SCRIPT % (_CULL_DEFORMATION,), str(path)],
-                   env=env, check=True, timeout=300)
+                   env=env, check=True, timeout=300, cwd=_REPO)
     ref = np.load(path)
     for seed, n, d in ((4, 80, None), (1, 200, None), (6, 120, _CULL_DEFORMATION)):
         obj, _, _ = W.random_scene(seed, max_prims=n)
