@@ -155,15 +155,18 @@ def test_two_pass_by_job_size(ctx, monkeypatch):
 
 
 def test_two_pass_only_where_it_applies(ctx, monkeypatch):
-    """Audit, the hierarchical integrator, FP64 and 4-sample blocks (24
-    nodes and up) keep the one-pass kernels."""
+    """Audit, FP64 and, for the simple integrator, 4-sample blocks (24 nodes
+    and up) keep the one-pass kernels; the guarded hierarchical march takes
+    two passes at any block size."""
     monkeypatch.setenv("XRP_TWO_PASS", "1")
     w = W.csg(resolution=48, n_views=1)
     sc = xrp_scene(w.obj)
     _, st = X.render(ctx, sc, w.angles, 48, w.ds, audit=True)
     assert st["two_pass"] == 0
-    _, st = X.render(ctx, sc, w.angles, 48, 10 * w.ds, integrator="hierarchical")
+    _, st = X.render(ctx, sc, w.angles, 48, 10 * w.ds, integrator="hierarchical", audit=True)
     assert st["two_pass"] == 0
+    _, st = X.render(ctx, sc, w.angles, 48, 10 * w.ds, integrator="hierarchical")
+    assert st["two_pass"] == 1
     _, st = X.render(ctx, sc, w.angles, 48, w.ds, precision="fp64")
     assert st["two_pass"] == 0
     sc.close()
@@ -171,6 +174,8 @@ def test_two_pass_only_where_it_applies(ctx, monkeypatch):
     sc = xrp_scene(obj)
     _, st = X.render(ctx, sc, w.angles, 48, sc.auto_ds())
     assert st["block"] == 4 and st["two_pass"] == 0
+    _, st = X.render(ctx, sc, w.angles, 48, sc.auto_ds(), integrator="hierarchical")
+    assert st["two_pass"] == 1
     sc.close()
 
 

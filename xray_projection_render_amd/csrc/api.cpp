@@ -764,6 +764,12 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       mp.cull = 1;
       std::memcpy(mp.M, cM, sizeof cM);
       std::memcpy(mp.t, ct, sizeof ct);
+      // a culled child stays further than the guard band from every FP32
+      // sample position, so it changes no FP32 density (the two-pass
+      // hierarchical march evaluates listed samples again without culling)
+      const double pad = 2.0 * delta * (1.0 + 1e-6);
+      mp.cull_pad = (float)pad;
+      if ((double)mp.cull_pad < pad) mp.cull_pad = std::nextafter(mp.cull_pad, INFINITY);
     }
   }
   mp.n_views = (int)job.views.size();
@@ -852,16 +858,17 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   const bool vox_kernel = pl.vox_kernel;
   const int layout = pl.layout;
   const double delta = pl.delta;
-  // Guarded simple march at block 1 or 2 in two passes (march.hip
-  // march_try_kernel), for jobs of 2^21 rays and up (below, the two extra
+  // Guarded simple march at block 1 or 2, and the guarded hierarchical
+  // march, in two passes (march.hip march_try_kernel,
+  // march_hier_try_kernel), for jobs of 2^21 rays and up (below, the two extra
   // launches cost more than they save: CSG 8 x 256^2 0.107 against 0.097
   // ms).  A launch covers at most 2^31 rays, which the lists name by 32-bit
   // index.  Not under audit, nor without allocation.
   const int kMaxGridY = 65535;
+  const bool hier = job.integrator == XRP_INTEGRATE_HIERARCHICAL;
   const bool two_pass = !fixed && !one_pass && !sc->one_pass &&
-                        precision == XRP_PRECISION_GUARDED && !vox_kernel &&
-                        job.integrator == XRP_INTEGRATE_SIMPLE && !job.audit && !pl.mp.queue &&
-                        (pl.mp.block == 1 || pl.mp.block == 2) &&
+                        precision == XRP_PRECISION_GUARDED && !vox_kernel && !job.audit &&
+                        (hier || (!pl.mp.queue && (pl.mp.block == 1 || pl.mp.block == 2))) &&
                         ((size_t)nv * res * res >= ((size_t)1 << 21) || two_pass_forced()) &&
                         two_pass_enabled();
   int vpc = two_pass ? (int)std::min<size_t>(kMaxGridY, ((size_t)1 << 31) / ((size_t)res * res))
@@ -930,16 +937,17 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   if (b_ctr) std::memset(hb + o_ctr, 0, b_ctr);
   // the lists, sized for the largest launch (XRP_DEFER_CAP and
   // XRP_REDO_CAP shorten them, for tests: 1 and up)
-  TwoPassCaps caps = two_pass_caps(two_pass ? (size_t)std::min(vpc, nv) * res * res : 0);
+  TwoPassCaps caps = two_pass_caps(two_pass ? (size_t)std::min(vpc, nv) * res * res : 0, hier);
   if (const char* e = std::getenv("XRP_DEFER_CAP"))
     caps.defer = (uint32_t)std::min<long>(caps.defer, std::max(1L, std::atol(e)));
   if (const char* e = std::getenv("XRP_REDO_CAP"))
     caps.redo = (uint32_t)std::min<long>(caps.redo, std::max(1L, std::atol(e)));
   // guard-band samples a ray may defer (XRP_DEFER_MAX, fewer for tests: 0
   // and up)
-  uint32_t defer_max = kDeferMax;
+  const uint32_t defer_lim = hier ? kHierDeferMax : kDeferMax;
+  uint32_t defer_max = defer_lim;
   if (const char* e = std::getenv("XRP_DEFER_MAX"))
-    defer_max = (uint32_t)std::min<long>(kDeferMax, std::max(0L, std::atol(e)));
+    defer_max = (uint32_t)std::min<long>(defer_lim, std::max(0L, std::atol(e)));
   if (two_pass) HIPCHK(ctx->redo.ensure(two_pass_bytes(caps)));
   HIPCHK(hipMemcpyAsync(db, hb, total, hipMemcpyHostToDevice, ctx->stream));
   // pageable staging: wait for the copy, so that no early return below can

@@ -132,7 +132,7 @@ __device__ __forceinline__ void sphere_blk(const DevNode& n, const PBlock<S>& p,
   }
 }
 
-template <int S, bool DIST>
+template <int S, bool DIST, bool G = false>
 __device__ __forceinline__ void box_blk(const DevNode& n, const PBlock<S>& p,
                                         double (&v)[S], unsigned& amb, float& dist) {
   const float cx = n.f[0], cy = n.f[1], cz = n.f[2], hx = n.f[3], hy = n.f[4], hz = n.f[5];
@@ -145,7 +145,7 @@ __device__ __forceinline__ void box_blk(const DevNode& n, const PBlock<S>& p,
     float az = fabsf(p.z[s] - cz) - hz;
     bool out = ax >= g0 || ay >= g1 || az >= g2;
     bool in = ax < -g0 && ay < -g1 && az < -g2;
-    v[s] = in ? rho : 0.0;
+    v[s] = (G ? ax < 0.0f && ay < 0.0f && az < 0.0f : in) ? rho : 0.0;
     amb |= (!(in || out) ? 1u : 0u) << s;
     if constexpr (DIST) {
       // outside: the largest face excess; inside: the nearest face
@@ -154,7 +154,7 @@ __device__ __forceinline__ void box_blk(const DevNode& n, const PBlock<S>& p,
   }
 }
 
-template <int S, bool DIST>
+template <int S, bool DIST, bool G = false>
 __device__ __forceinline__ void pped_blk(const DevNode& n, const PBlock<S>& p,
                                          double (&v)[S], unsigned& amb, float& dist) {
   const float ox = n.f[0], oy = n.f[1], oz = n.f[2];
@@ -172,7 +172,8 @@ __device__ __forceinline__ void pped_blk(const DevNode& n, const PBlock<S>& p,
                t >= 1.0f + g2;
     bool in = u > g0 && u < 1.0f - g0 && vv > g1 && vv < 1.0f - g1 && t > g2 &&
               t < 1.0f - g2;
-    v[s] = in ? rho : 0.0;
+    const bool guess = u > 0.0f && u < 1.0f && vv > 0.0f && vv < 1.0f && t > 0.0f && t < 1.0f;
+    v[s] = (G ? guess : in) ? rho : 0.0;
     amb |= (!(in || out) ? 1u : 0u) << s;
     if constexpr (DIST) {
       if (s == S - 1) {
@@ -186,7 +187,7 @@ __device__ __forceinline__ void pped_blk(const DevNode& n, const PBlock<S>& p,
   }
 }
 
-template <int S, bool DIST>
+template <int S, bool DIST, bool G = false>
 __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
                                         double (&v)[S], unsigned& amb, float& dist) {
   const float px = n.f[0], py = n.f[1], pz = n.f[2];
@@ -204,7 +205,7 @@ __device__ __forceinline__ void cyl_blk(const DevNode& n, const PBlock<S>& p,
     bool cap_in = t > gt && t < vv - gt;
     bool in = cap_in && q < -gq;
     bool out = cap_out || q >= gq;
-    v[s] = in ? rho : 0.0;
+    v[s] = (G ? t >= 0.0f && t <= vv && q < 0.0f : in) ? rho : 0.0;
     amb |= (!(in || out) ? 1u : 0u) << s;
     if constexpr (DIST) {
       if (s == S - 1) {
@@ -293,7 +294,7 @@ struct Cull {
   int n = -1;
 };
 
-template <int D, int S, bool DIST>
+template <int D, int S, bool DIST, bool G = false>
 __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
                                          unsigned& amb_out, float& dist,
@@ -301,7 +302,9 @@ __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
 
 // Tessellation wrap (objects.go:568-582) per sample, then the unit cell's
 // collection for the samples that passed the bounds and seam guards.
-template <int D, int S, bool DIST>
+// G (a best guess for samples inside a guard band): those inside the
+// bounds by the FP32 test are evaluated in the cell the FP32 wrap picks.
+template <int D, int S, bool DIST, bool G = false>
 __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
                                          unsigned& amb_out, float& dist) {
@@ -342,10 +345,13 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
       q.vz = p.vz;
     }
     if (!o) {
-      if (!in || (seam && !seamless))
+      if (!in || (seam && !seamless)) {
         amb |= 1u << s;
-      else
+        if (G && x >= b0 && x <= b1 && y >= b2 && y <= b3 && z >= b4 && z <= b5)
+          go |= 1u << s;
+      } else {
         go |= 1u << s;
+      }
     }
     if constexpr (DIST) {
       if (s == S - 1) {
@@ -360,7 +366,7 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
   go &= live;
   amb_out |= amb;
   if constexpr (D + 1 <= kMaxDepth) {
-    if (go) coll_blk<D + 1, S, DIST>(N, n.first, q, go, out, amb_out, dist);
+    if (go) coll_blk<D + 1, S, DIST, G>(N, n.first, q, go, out, amb_out, dist);
   } else {
     amb_out |= go;
   }
@@ -436,23 +442,23 @@ __device__ __forceinline__ void tess_blk(const DevNode* __restrict__ N, int i,
   }
 }
 
-template <int D, int S, bool DIST>
+template <int D, int S, bool DIST, bool G = false>
 __device__ __forceinline__ void leaf_blk(const DevNode* __restrict__ N, int c,
                                          const PBlock<S>& p, unsigned live, double (&v)[S],
                                          unsigned& amb, float& dist) {
   const DevNode& k = N[c];
   switch (k.kind) {
     case K_SPHERE: sphere_blk<S, DIST>(k, p, v, amb, dist); return;
-    case K_BOX: box_blk<S, DIST>(k, p, v, amb, dist); return;
-    case K_PPED: pped_blk<S, DIST>(k, p, v, amb, dist); return;
-    case K_CYL: cyl_blk<S, DIST>(k, p, v, amb, dist); return;
+    case K_BOX: box_blk<S, DIST, G>(k, p, v, amb, dist); return;
+    case K_PPED: pped_blk<S, DIST, G>(k, p, v, amb, dist); return;
+    case K_CYL: cyl_blk<S, DIST, G>(k, p, v, amb, dist); return;
     case K_GYROID: gyroid_blk<S, DIST>(k, p, v, amb, dist); return;
     case K_TESS:
       // Tessellations below the first level (a tessellation inside a
       // tessellated cell) are left to the FP64 evaluator: the blocked
       // walk keeps one level of cell state in registers.
       if constexpr (D == 0) {
-        tess_blk<D, S, DIST>(N, c, p, live, v, amb, dist);
+        tess_blk<D, S, DIST, G>(N, c, p, live, v, amb, dist);
       } else {
 #pragma unroll
         for (int s = 0; s < S; ++s) v[s] = 0.0;
@@ -476,8 +482,10 @@ __device__ __forceinline__ void leaf_blk(const DevNode* __restrict__ N, int c,
 }
 
 // objects.go:422-438 per sample: sum then clamp to [0, 1]; greedy returns
-// the first positive child unclamped.
-template <int D, int S, bool DIST>
+// the first positive child unclamped.  A sample whose child is inside a
+// guard band is flagged and left (its value is not used), or with G (a
+// best guess) summed on with the child's FP32 value.
+template <int D, int S, bool DIST, bool G>
 __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
                                          const PBlock<S>& p, unsigned live, double (&out)[S],
                                          unsigned& amb_out, float& dist, const Cull& cl) {
@@ -500,13 +508,13 @@ __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
     unsigned a = 0;
     // A greedy collection's result is fixed by its children up to the
     // first positive one, so the distances of the children walked bound it.
-    leaf_blk<D, S, DIST>(N, c, p, act, v, a, dist);
+    leaf_blk<D, S, DIST, G>(N, c, p, act, v, a, dist);
     a &= act;
     amb |= a;
-    done |= a;
+    if (!G) done |= a;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (((act & ~a) >> s) & 1u) {
+      if (((act & (G ? ~0u : ~a)) >> s) & 1u) {
         if (greedy && v[s] > 0.0) {
           out[s] = v[s];
           hit |= 1u << s;
@@ -519,7 +527,7 @@ __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
   }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (((live & ~amb & ~hit) >> s) & 1u) {
+    if (((live & (G ? ~0u : ~amb) & ~hit) >> s) & 1u) {
       double d = out[s];
       out[s] = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
     }
@@ -530,22 +538,23 @@ __device__ __forceinline__ void coll_blk(const DevNode* __restrict__ N, int i,
 // dist (DIST only) starts at +inf in the caller and is lowered by every
 // node the last sample visits.
 // cl: the wave's culled child list when the root is a collection.
-template <int S, bool DIST = false>
+// G: samples inside a guard band get a best guess, their FP32 value.
+template <int S, bool DIST = false, bool G = false>
 __device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PBlock<S>& p,
                                          unsigned live, double (&out)[S], unsigned& amb,
                                          float& dist, const Cull& cl = Cull()) {
   if (N[0].kind == K_COLL)
-    coll_blk<0, S, DIST>(N, 0, p, live, out, amb, dist, cl);
+    coll_blk<0, S, DIST, G>(N, 0, p, live, out, amb, dist, cl);
   else
-    leaf_blk<0, S, DIST>(N, 0, p, live, out, amb, dist);
+    leaf_blk<0, S, DIST, G>(N, 0, p, live, out, amb, dist);
   amb &= live;
 }
-template <int S>
+template <int S, bool G = false>
 __device__ __forceinline__ void root_blk(const DevNode* __restrict__ N, const PBlock<S>& p,
                                          unsigned live, double (&out)[S], unsigned& amb,
                                          const Cull& cl = Cull()) {
   float unused = 0.0f;
-  root_blk<S, false>(N, p, live, out, amb, unused, cl);
+  root_blk<S, false, G>(N, p, live, out, amb, unused, cl);
 }
 
 

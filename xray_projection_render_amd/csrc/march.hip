@@ -69,14 +69,91 @@ march_kernel(MarchParams p) {
   flush_counts(p.stats, rc);
 }
 
+// End of a first-pass wave of the two-pass marches; every lane calls it.
+// A ray with nothing deferred stores its pixel (SPEC: every ray not stopped
+// does, its listed samples to be checked, and its counts follow its items
+// in defer_k: samples, fine samples, settled samples).  The rays with
+// deferred items are appended, with T and the items (ts.ks), to the
+// resolve pass's lists in lane order (an exclusive scan of the counts over
+// the wave, one atomic per wave and list).  The stopped ones, and a wave's deferring rays when
+// the lists are full, go to p.redo_list.  The counts of a ray go to rc only
+// when it is not marched again.
+template <bool SPEC = false>
+__device__ __forceinline__ void try_finish(const MarchParams& p, int v, int i, int j,
+                                           bool valid, double T, TryState& ts,
+                                           RayCounts& rc) {
+  const int lane = threadIdx.x & 63;
+  const unsigned id = ((unsigned)v * (unsigned)p.res + (unsigned)i) * (unsigned)p.res +
+                      (unsigned)j;
+  if (valid && !ts.aborted && (SPEC || ts.cnt == 0)) store_pixel(p, v, i, j, T);
+  bool defer = valid && !ts.aborted && ts.cnt > 0;
+  const unsigned kc = ts.cnt + (SPEC ? 3u : 0u);  // words in defer_k
+  const unsigned long long dm = __ballot(defer);
+  if (dm) {
+    unsigned incl = defer ? kc : 0u;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned t = (unsigned)__shfl_up((int)incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const unsigned total = (unsigned)__shfl((int)incl, 63, 64);
+    unsigned eb = 0, kb = 0;
+    if (lane == 0) {
+      eb = atomicAdd(p.counts + 1, (unsigned)__popcll(dm));
+      kb = atomicAdd(p.counts + 2, total);
+    }
+    eb = (unsigned)__shfl((int)eb, 0, 64);
+    kb = (unsigned)__shfl((int)kb, 0, 64);
+    const bool fits = eb + (unsigned)__popcll(dm) <= p.defer_cap && kb + total <= p.k_cap;
+    if (defer && fits) {
+      const unsigned e = eb + (unsigned)__popcll(dm & __lanemask_lt());
+      const unsigned k0 = kb + incl - kc;
+      p.defer_T[e] = T;
+      p.defer_ray[e] = id;
+      p.defer_cnt[e] = ts.cnt;
+      p.defer_kfirst[e] = k0;
+      for (unsigned c = 0; c < ts.cnt; ++c) p.defer_k[k0 + c] = ts.ks[c];
+      if (SPEC) {
+        p.defer_k[k0 + ts.cnt] = ts.samples;
+        p.defer_k[k0 + ts.cnt + 1] = ts.fine;
+        p.defer_k[k0 + ts.cnt + 2] = ts.skipped;
+      }
+    } else if (defer) {
+      // the lists are full: the wave's rays go to the redo list, and its
+      // entries below the cap are marked empty for the resolve pass
+      const unsigned e = eb + (unsigned)__popcll(dm & __lanemask_lt());
+      if (e < p.defer_cap) p.defer_cnt[e] = 0u;
+      ts.aborted = true;
+    }
+  }
+  if (valid && !ts.aborted) {
+    rc.samples += ts.samples;
+    rc.skipped += ts.skipped;
+    rc.fine += ts.fine;
+  }
+  const unsigned long long m = __ballot(valid && ts.aborted);
+  if (m) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(p.counts, (unsigned)__popcll(m));
+    base = (unsigned)__shfl((int)base, 0, 64);
+    const unsigned e = base + (unsigned)__popcll(m & __lanemask_lt());
+    if (valid && ts.aborted) {
+      if (e < p.redo_cap)
+        p.redo_list[e] = id;
+      else
+        p.counts[3] = 1u;  // no room: the caller renders again in one pass
+    }
+  }
+  flush_counts(p.stats, rc);
+}
+
 // Two-pass guarded simple march, first pass: every ray marched without the
 // FP64 evaluator (march_T_guarded_inline<B, true>), one launch over all
 // views like the one-pass kernel.  A ray with no guard-band sample stores
 // its pixel; one with a few is appended, with its T and the deferred
 // sample indices, to the resolve pass's lists; one with more, or finding
-// the lists full, to p.redo_list for march_redo_kernel (one atomic per
-// wave and list).  Without the FP64 path in the loop no lane waits on
-// another's FP64 re-evaluation.
+// the lists full, to p.redo_list for march_redo_kernel.  Without the FP64
+// path in the loop no lane waits on another's FP64 re-evaluation.
 template <int B>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
 march_try_kernel(MarchParams p) {
@@ -95,61 +172,119 @@ march_try_kernel(MarchParams p) {
   TryState ts;
   ts.ks = kbuf + lane * kDeferMax;
   const double T = valid ? march_T_guarded_inline<B, true>(p, v, i, j, rc, cl, &ts) : 0.0;
-  const unsigned id = ((unsigned)v * (unsigned)p.res + (unsigned)i) * (unsigned)p.res +
-                      (unsigned)j;
-  if (valid && !ts.aborted && ts.cnt == 0) store_pixel(p, v, i, j, T);
-  // rays with deferred samples: entries and their sample indices, in lane
-  // order (an exclusive scan of the counts over the wave)
-  bool defer = valid && !ts.aborted && ts.cnt > 0;
-  const unsigned long long dm = __ballot(defer);
-  if (dm) {
-    unsigned incl = defer ? ts.cnt : 0u;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const unsigned t = (unsigned)__shfl_up((int)incl, off, 64);
-      if (lane >= off) incl += t;
+  try_finish(p, v, i, j, valid, T, ts, rc);
+}
+
+// Two-pass hierarchical march, first pass (march_T_hier<PREC_GUARDED, 1,
+// true>): no FP64 evaluator in the kernel.  Guard-band samples keep their
+// FP32 values; every ray not stopped stores its pixel, and the rays with
+// such samples (at most p.defer_max <= kHierDeferMax) list them for
+// march_hier_check_kernel.  Stopped rays go to march_hier_redo_kernel.
+template <int BC, int BF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BC == 4 ? 2 : 3))) void
+march_hier_try_kernel(MarchParams p) {
+  __shared__ uint32_t kbuf[64 * kHierDeferMax];
+  const int lane = threadIdx.x;
+  const int TI = p.tile_j32 ? 2 : march_tile_i(INT_HIERARCHICAL);
+  const int TJ = p.tile_j32 ? 32 : march_tile_j(INT_HIERARCHICAL);
+  const int tiles_j = (p.res + TJ - 1) / TJ;
+  const int ti = blockIdx.x / tiles_j, tj = blockIdx.x - ti * tiles_j;
+  const int i = ti * TI + lane / TJ;
+  const int j = tj * TJ + lane % TJ;
+  const int v = blockIdx.y;
+  const bool valid = i < p.res && j < p.res;
+  RayCounts rc;
+  const Cull cl = wave_cull<1>(p, v, i, j, valid);
+  TryState ts;
+  ts.ks = kbuf + lane * kHierDeferMax;
+  const double T = march_T_hier<PREC_GUARDED, 1, true, BC, BF>(p, v, i, j, valid, rc, cl, &ts);
+  try_finish<true>(p, v, i, j, valid, T, ts, rc);
+}
+
+// Check pass of the two-pass hierarchical march: one lane per entry
+// (grid-stride).  Each listed sample, (nn << 4) | m, is at R[nn] for m = 15
+// and otherwise at R[nn-1] + DS/10 plus m more additions of DS/10 (the
+// m-th sample of step nn's fine walk, main.go:180-186).  Its FP32 value,
+// which the first pass used, is computed again (it depends on the position
+// alone) and compared with its FP64 density; a ray with any that differs
+// goes to the redo list, whose pass overwrites its pixel, and its first
+// pass counts (after its items) are taken back from the stats.
+__global__ __launch_bounds__(64) void march_hier_check_kernel(MarchParams p) {
+  const unsigned n = min(p.counts[1], p.defer_cap), nk = min(p.counts[2], p.k_cap);
+  const unsigned rr = (unsigned)p.res * (unsigned)p.res;
+  const double dsf = p.ds_fine;
+  const Cull none;
+  unsigned long long reeval = 0, rays = 0, back_s = 0, back_f = 0, back_k = 0;
+  for (unsigned e = blockIdx.x * 64u + threadIdx.x; e < n; e += gridDim.x * 64u) {
+    const unsigned cnt = p.defer_cnt[e], k0 = p.defer_kfirst[e];
+    if (cnt == 0u || k0 + cnt + 3u > nk) continue;
+    const unsigned id = p.defer_ray[e];
+    const unsigned v = id / rr, pix = id - v * rr;
+    const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
+    const Ray r = make_ray(p, p.views[p.view_offset + (int)v], i, j);
+    bool same = true;
+    for (unsigned c = 0; c < cnt && same; ++c) {
+      const unsigned item = p.defer_k[k0 + c], nn = item >> 4, m = item & 15u;
+      double s[1];
+      if (m == 15u) {
+        s[0] = p.s_table[nn];
+      } else {
+        s[0] = (nn == 0 ? p.smin : p.s_table[nn - 1]) + dsf;
+        for (unsigned q = 0; q < m; ++q) s[0] += dsf;
+      }
+      double f32[1];
+      unsigned amb;
+      unsigned long long unused = 0;
+      eval_along<PREC_GUARDED, 1, false, true>(p, r.o, r.d, s, 1u, f32, unused, unused, none,
+                                               amb);
+      const double ev = density64(p, r.o[0] + r.d[0] * s[0], r.o[1] + r.d[1] * s[0],
+                                  r.o[2] + r.d[2] * s[0]);
+      same = ev == f32[0];
     }
-    const unsigned total = (unsigned)__shfl((int)incl, 63, 64);
-    unsigned eb = 0, kb = 0;
-    if (lane == 0) {
-      eb = atomicAdd(p.counts + 1, (unsigned)__popcll(dm));
-      kb = atomicAdd(p.counts + 2, total);
-    }
-    eb = (unsigned)__shfl((int)eb, 0, 64);
-    kb = (unsigned)__shfl((int)kb, 0, 64);
-    const bool fits = eb + (unsigned)__popcll(dm) <= p.defer_cap && kb + total <= p.k_cap;
-    if (defer && fits) {
-      const unsigned e = eb + (unsigned)__popcll(dm & __lanemask_lt());
-      const unsigned k0 = kb + incl - ts.cnt;
-      p.defer_T[e] = T;
-      p.defer_ray[e] = id;
-      p.defer_cnt[e] = ts.cnt;
-      p.defer_kfirst[e] = k0;
-      for (unsigned c = 0; c < ts.cnt; ++c) p.defer_k[k0 + c] = ts.ks[c];
-    } else if (defer) {
-      // the lists are full: the wave's rays go to the redo list, and its
-      // entries below the cap are marked empty for the resolve pass
-      const unsigned e = eb + (unsigned)__popcll(dm & __lanemask_lt());
-      if (e < p.defer_cap) p.defer_cnt[e] = 0u;
-      ts.aborted = true;
-    }
-  }
-  if (valid && !ts.aborted) {
-    rc.samples += ts.samples;
-    rc.skipped += ts.skipped;
-  }
-  const unsigned long long m = __ballot(valid && ts.aborted);
-  if (m) {
-    unsigned base = 0;
-    if (lane == 0) base = atomicAdd(p.counts, (unsigned)__popcll(m));
-    base = (unsigned)__shfl((int)base, 0, 64);
-    const unsigned e = base + (unsigned)__popcll(m & __lanemask_lt());
-    if (valid && ts.aborted) {
-      if (e < p.redo_cap)
-        p.redo_list[e] = id;
+    if (same) {
+      reeval += cnt;
+      ++rays;
+    } else {
+      back_s += p.defer_k[k0 + cnt];
+      back_f += p.defer_k[k0 + cnt + 1];
+      back_k += p.defer_k[k0 + cnt + 2];
+      const unsigned at = atomicAdd(p.counts, 1u);
+      if (at < p.redo_cap)
+        p.redo_list[at] = id;
       else
         p.counts[3] = 1u;  // no room: the caller renders again in one pass
     }
+  }
+  if (p.stats) {
+    const unsigned long long a = wave_sum(reeval), b = wave_sum(rays), c = wave_sum(back_s),
+                             d = wave_sum(back_f), f = wave_sum(back_k);
+    if (threadIdx.x == 0) {
+      if (a) atomicAdd(p.stats + ST_REEVAL, a);
+      if (b) atomicAdd(p.stats + ST_DEFERRED, b);
+      // the counts of rays marched again, taken back (mod 2^64)
+      if (c) atomicAdd(p.stats + ST_SAMPLES, 0ull - c);
+      if (d) atomicAdd(p.stats + ST_FINE, 0ull - d);
+      if (f) atomicAdd(p.stats + ST_SKIPPED, 0ull - f);
+    }
+  }
+}
+
+// Second pass of the two-pass hierarchical march: the listed rays (stopped
+// in the first pass, or with a listed sample whose FP32 value was not
+// exact), 64 per wave in list order (grid-stride), marched again by the
+// one-pass march.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void
+march_hier_redo_kernel(MarchParams p) {
+  const unsigned n = min(p.counts[0], p.redo_cap);
+  const unsigned rr = (unsigned)p.res * (unsigned)p.res;
+  RayCounts rc;
+  for (unsigned base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+    const unsigned e = base + threadIdx.x;
+    const bool valid = e < n;
+    const unsigned id = valid ? p.redo_list[e] : 0u;
+    const unsigned v = id / rr, pix = id - v * rr;
+    const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
+    march_ray<PREC_GUARDED, INT_HIERARCHICAL, 1>(p, (int)v, i, j, valid, rc);
   }
   flush_counts(p.stats, rc);
 }
@@ -423,7 +558,21 @@ int launch_march(const MarchParams& p, int precision, int integrator,
       hipLaunchKernelGGL((march_kernel<PREC_FP64, INT_SIMPLE, kBlock>), grid, block, 0,
                          s, p);
   } else {
-    if (integrator == INT_HIERARCHICAL)
+    if (integrator == INT_HIERARCHICAL && p.redo_list && !p.audit) {
+      const dim3 grid3(std::min((p.defer_cap + 63u) / 64u, 8192u));
+      const dim3 grid2(std::min((p.redo_cap + 63u) / 64u, 3072u));
+      // coarse samples per node visit as the simple march's block (1 where
+      // gyroids reach far, 4 on large collections), 2 per fine visit on
+      // small trees
+      if (p.block == 1)
+        hipLaunchKernelGGL((march_hier_try_kernel<1, 2>), grid, block, 0, s, p);
+      else if (p.block == 2)
+        hipLaunchKernelGGL((march_hier_try_kernel<2, 2>), grid, block, 0, s, p);
+      else
+        hipLaunchKernelGGL((march_hier_try_kernel<4, 4>), grid, block, 0, s, p);
+      hipLaunchKernelGGL(march_hier_check_kernel, grid3, block, 0, s, p);
+      hipLaunchKernelGGL(march_hier_redo_kernel, grid2, block, 0, s, p);
+    } else if (integrator == INT_HIERARCHICAL)
       hipLaunchKernelGGL((march_kernel<PREC_GUARDED, INT_HIERARCHICAL, kBlock>), grid,
                          block, 0, s, p);
     else if (p.redo_list && !p.audit && !p.queue && (p.block == 1 || p.block == 2)) {

@@ -89,10 +89,14 @@ struct MarchParams {
   // Per-scene march kernel: rays of a wave as 2 x 32 pixels (32 along the
   // image's contiguous axis j) instead of 32 x 2.
   int32_t tile_j32;
-  // Guarded simple march at block 1 or 2 in two passes (march.hip) when
-  // redo_list is set.  The first pass (march_try_kernel) marches every ray
-  // without the FP64 evaluator and defers its guard-band samples (at most
-  // defer_max, <= kDeferMax; their indices wait in LDS).  At the end of a
+  // Guarded simple march at block 1 or 2, and the guarded hierarchical
+  // march, in two passes (march.hip) when redo_list is set.  The first pass
+  // (march_try_kernel) marches every ray without the FP64 evaluator and
+  // defers its guard-band samples (at most defer_max, <= kDeferMax; their
+  // indices wait in LDS).  The hierarchical first pass instead goes on with
+  // their FP32 values, stores the pixel and lists the samples, at most
+  // kHierDeferMax, as (coarse index << 4) | position in the fine walk (15
+  // for the coarse sample itself), for a pass that checks them in FP64.  At the end of a
   // wave its rays with deferred samples are appended (one atomic per wave)
   // as entries e < defer_cap: the ray (v * res^2 + i * res + j, v counted
   // from view_offset) to defer_ray[e], its T without those samples to
@@ -114,6 +118,11 @@ struct MarchParams {
   uint32_t* defer_k;
   uint32_t redo_cap, defer_cap, k_cap;
   uint32_t defer_max;
+  // Culling: radius added to every child's bounding sphere, twice the
+  // object-space position error, so that a culled child is outside every
+  // sample's FP32 position by more than its guard band (the FP32 density
+  // of a sample is then the same with or without the child).
+  float cull_pad;
 };
 
 // Guard-band samples a ray may defer to the resolve pass (the first pass
@@ -121,18 +130,26 @@ struct MarchParams {
 // by one wave with few rays, which takes long: on the sheared gyroid
 // lattice 8 left 78 rays of 33.5 M to the redo pass (0.9 ms).
 constexpr uint32_t kDeferMax = 32;
+// Guard-band samples a ray the first pass of the two-pass hierarchical
+// march may list (LDS: 3.5 KiB per wave beside the toggle queue and the
+// cull list, so that 12 one-wave workgroups fit a CU's 160 KiB).
+constexpr uint32_t kHierDeferMax = 14;
 // Two-pass list sizes for a launch of `rays` rays: room for every ray (two
 // sample indices each) up to 2^22 rays; beyond, a ray entry per 8 rays, a
 // sample index per 4 and a redo entry per 64, but never less than for 2^22.
 struct TwoPassCaps {
   uint32_t redo, defer, k;
 };
-inline TwoPassCaps two_pass_caps(size_t rays) {
+// The hierarchical march's guard-band samples include fine samples, which
+// lie at surfaces, so more rays have some: beyond 2^22 rays a ray entry per
+// 2 rays, two sample entries per ray and a redo entry per 16.
+inline TwoPassCaps two_pass_caps(size_t rays, bool hierarchical = false) {
   const size_t small = rays < ((size_t)1 << 22) ? rays : ((size_t)1 << 22);
   auto cap = [&](size_t all, size_t share) {
     const size_t c = all > share ? all : share;
     return (uint32_t)(c < 64 ? 64 : c);
   };
+  if (hierarchical) return {cap(small, rays / 16), cap(small, rays / 2), cap(2 * small, 2 * rays)};
   return {cap(small, rays / 64), cap(small, rays / 8), cap(2 * small, rays / 4)};
 }
 // Bytes of the two-pass buffers for those sizes (each array 256-byte aligned).

@@ -92,7 +92,7 @@ __device__ __forceinline__ Cull wave_cull(const MarchParams& p, int v, int i, in
                    w2 = (double)ch.bsph[2] - o[2];
       const double t = (w0 * d[0] + w1 * d[1] + w2 * d[2]) * inv;
       const double q0 = w0 - t * d[0], q1 = w1 - t * d[1], q2 = w2 - t * d[2];
-      const double rr = (double)ch.bsph[3];
+      const double rr = (double)ch.bsph[3] + (double)p.cull_pad;
       keep = q0 * q0 + q1 * q1 + q2 * q2 <= rr * rr;
     }
     if (__ballot(keep)) {
@@ -139,8 +139,9 @@ __device__ __forceinline__ double march_T_simple64(const MarchParams& p, int v, 
 // State of one ray in the first pass of the two-pass march.
 struct TryState {
   uint32_t* ks = nullptr;  // the lane's kDeferMax sample indices (LDS)
-  unsigned cnt = 0;        // guard-band samples deferred
+  unsigned cnt = 0;        // guard-band samples deferred (hierarchical: listed)
   unsigned samples = 0, skipped = 0;  // the ray's counts, when it was not stopped
+  unsigned fine = 0;                  // hierarchical: fine samples of its own steps
   bool aborted = false;
 };
 
@@ -505,26 +506,37 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
 
 constexpr int kHierCap = 512;   // queued toggles per wave
 
+// Per-wave toggle queue.  TRY (the first pass of the two-pass march) also
+// keeps each walk's guard-band positions and counts for the owning lane.
+template <bool TRY>
 struct HierQueue {
-  double sum[kHierCap];     // fine sum of the step: sum of (rho * dm) * DS/10
-  int32_t n[kHierCap];      // coarse index of the step's right endpoint
-  int16_t link[kHierCap];   // the owning lane's next entry, -1 for none
-  uint8_t owner[kHierCap];  // lane whose ray the step belongs to
+  double sum[kHierCap];               // fine sum of the step: sum of (rho * dm) * DS/10
+  int32_t n[kHierCap];                // coarse index of the step's right endpoint
+  int16_t link[kHierCap];             // the owning lane's next entry, -1 for none
+  uint16_t ambm[TRY ? kHierCap : 1];  // bit m: the walk's m-th sample was in a guard band
+  uint8_t cnt[TRY ? kHierCap : 1];    // the walk's fine samples | settled ones << 4
+  uint8_t owner[kHierCap];            // lane whose ray the step belongs to
 };
 
 // Densities (dm not applied) at s[q] along the ray o + s d, for the samples
 // in `live`.  Guarded: FP32 membership with guard bands over the block and
 // FP64 re-evaluation of the ambiguous samples (all of them under audit),
-// through one FP64 evaluator call site.  With DIST (guarded only), dist is
-// the last sample's safe radius (eval_block.hpp) and last_amb whether that
-// sample was ambiguous.
-template <int PREC, int B, bool DIST = false>
+// through one FP64 evaluator call site; amb is the mask of the ambiguous
+// ones.  TRY: no FP64 evaluator; the ambiguous samples get a best guess,
+// their FP32 value (root_blk with G).  With DIST (guarded only), dist is the last sample's safe radius
+// (eval_block.hpp) and last_amb whether that sample was ambiguous.  A
+// sample's value depends on its position alone (and not on the block, the
+// cull list, whose children are further than the guard band from the ray,
+// or DIST).
+template <int PREC, int B, bool DIST = false, bool TRY = false>
 __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&o)[3],
                                            const double (&d)[3], const double (&s)[B],
                                            unsigned live, double (&rho)[B],
                                            unsigned long long& n_reeval,
                                            unsigned long long& n_miss, const Cull& cl,
-                                           float* dist = nullptr, bool* last_amb = nullptr) {
+                                           unsigned& amb, float* dist = nullptr,
+                                           bool* last_amb = nullptr) {
+  amb = 0;
   if constexpr (PREC == PREC_FP64) {
 #pragma unroll
     for (int q = 0; q < B; ++q)
@@ -545,48 +557,136 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
       P.vz = (float)d[2];
     }
     deform_blk<B>(p.defs, p.n_defs, P, DIST && p.vel != 0);
-    unsigned amb = 0;
     if constexpr (DIST) {
-      root_blk<B, true>(p.nodes, P, live, rho, amb, *dist, cl);
+      root_blk<B, true, TRY>(p.nodes, P, live, rho, amb, *dist, cl);
       *last_amb = (amb >> (B - 1)) & 1u;
     } else {
-      root_blk<B>(p.nodes, P, live, rho, amb, cl);
+      root_blk<B, TRY>(p.nodes, P, live, rho, amb, cl);
     }
-    unsigned redo = p.audit ? live : amb;
-    while (redo) {
-      const int q = __builtin_ctz(redo);
-      redo &= redo - 1u;
-      double sq = s[0], rq = rho[0];
+    if constexpr (!TRY) {
+      unsigned redo = p.audit ? live : amb;
+      while (redo) {
+        const int q = __builtin_ctz(redo);
+        redo &= redo - 1u;
+        double sq = s[0], rq = rho[0];
 #pragma unroll
-      for (int t = 1; t < B; ++t)
-        if (q == t) {
-          sq = s[t];
-          rq = rho[t];
-        }
-      const double e = density64(p, o[0] + d[0] * sq, o[1] + d[1] * sq, o[2] + d[2] * sq);
-      if ((amb >> q) & 1u)
-        ++n_reeval;
-      else if (e != rq)
-        ++n_miss;
+        for (int t = 1; t < B; ++t)
+          if (q == t) {
+            sq = s[t];
+            rq = rho[t];
+          }
+        const double e = density64(p, o[0] + d[0] * sq, o[1] + d[1] * sq, o[2] + d[2] * sq);
+        if ((amb >> q) & 1u)
+          ++n_reeval;
+        else if (e != rq)
+          ++n_miss;
 #pragma unroll
-      for (int t = 0; t < B; ++t)
-        if (q == t) rho[t] = e;
+        for (int t = 0; t < B; ++t)
+          if (q == t) rho[t] = e;
+      }
     }
   }
+}
+
+// The fine walk of the step whose right endpoint is coarse sample nn
+// (main.go:180-188): left = R[nn-1] + DS/10, then left += DS/10 while left <
+// right, each sample adding (rho * dm) * DS/10.  Returns that sum, in
+// order; ambm gets the positions of the samples inside a guard band (bit m:
+// the walk's m-th sample), nf the walk's samples and nsk those a safe
+// radius settled.  TRY: no FP64 evaluator, those samples add their FP32
+// values.  BF: fine samples per node visit; the sum does not depend on it.
+template <int PREC, bool TRY = false, int BF = kHierBlock>
+__device__ __forceinline__ double fine_walk(const MarchParams& p, const double (&o)[3],
+                                            const double (&d)[3], int nn, const Cull& cl,
+                                            unsigned& nf, unsigned& nsk, unsigned& ambm,
+                                            unsigned long long& n_reeval,
+                                            unsigned long long& n_miss) {
+  constexpr int B = PREC == PREC_FP64 ? 1 : BF;
+  constexpr bool kDist = PREC == PREC_GUARDED;
+  const double dsf = p.ds_fine, dm = p.dm;
+  const double right = p.s_table[nn];
+  double left = (nn == 0 ? p.smin : p.s_table[nn - 1]) + dsf;
+  double S = 0.0;
+  nf = 0;
+  nsk = 0;
+  ambm = 0;
+  while (left < right) {
+    double s[B], rf[B];
+    unsigned live = 0;
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      s[q] = left;
+      if (left < right) {
+        live |= 1u << q;
+        left += dsf;
+      }
+    }
+    float dist = INFINITY;
+    bool last_amb = true;
+    unsigned a = 0;
+    eval_along<PREC, B, kDist, TRY>(p, o, d, s, live, rf, n_reeval, n_miss, cl, a, &dist,
+                                    &last_amb);
+    ambm |= a << nf;
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+      if ((live >> q) & 1u) S += (rf[q] * dm) * dsf;
+    nf += (unsigned)__builtin_popcount(live);
+    // Fine samples inside the last one's safe radius: same terms, same
+    // left += ds walk, no evaluation.
+    if constexpr (kDist) {
+      float f = dist * p.skip_inv * 10.0f;  // skip_inv is per DS = 10 ds
+      if (live == (1u << B) - 1u && !last_amb && f >= 1.0f) {
+        const double c = (rf[B - 1] * dm) * dsf;
+        for (int m = f < 64.0f ? (int)f : 64; m > 0 && left < right; --m) {
+          if constexpr (!TRY) {
+            if (p.audit) {
+              const double e2 = density64(p, o[0] + d[0] * left, o[1] + d[1] * left,
+                                          o[2] + d[2] * left);
+              if (e2 != rf[B - 1]) ++n_miss;
+            }
+          }
+          S += c;
+          left += dsf;
+          ++nf;
+          ++nsk;
+        }
+      }
+    }
+  }
+  return S;
 }
 
 // Optical depth T of pixel (i, j) of view v, hierarchical integrator.  Every
 // lane of the wave calls it together (lanes without a pixel pass valid =
 // false and still help drain the queue); T is meaningful for valid lanes.
-template <int PREC, int QW>
+// The coarse terms go into T and the fine sums, in coarse order, into Tf:
+// the result, T + Tf, does not depend on when the queue is drained (nor,
+// therefore, on which other rays share the wave or on how many samples are
+// settled).
+//
+// TRY: the first pass of the two-pass hierarchical march, without the FP64
+// evaluator.  Guard-band samples, coarse and fine, keep their FP32 values
+// (almost always the exact ones: the bands are worst-case bounds) and are
+// listed in ts->ks as (nn << 4) | m: the m-th sample of the fine walk of
+// step nn, m = 15 for coarse sample nn itself.  A later pass evaluates them
+// in FP64; where every listed value was exact, T + Tf is the one-pass
+// march's, bit for bit.  A ray with more than p.defer_max of them, or a walk
+// with 15 samples or more, stops (ts->aborted).  The ray's counts go to ts,
+// not rc.
+//
+// BC, BF: coarse and fine samples per node visit.  Every term is the exact
+// density of its sample added in order, so the sums do not depend on them.
+template <int PREC, int QW, bool TRY = false, int BC = kHierBlock, int BF = kHierBlock>
 __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int i, int j,
-                                               bool valid, RayCounts& rc, const Cull& cl) {
-  constexpr int B = PREC == PREC_FP64 ? 1 : kHierBlock;
-  __shared__ HierQueue queues[QW];
-  HierQueue& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
+                                               bool valid, RayCounts& rc, const Cull& cl,
+                                               TryState* ts = nullptr) {
+  constexpr int B = PREC == PREC_FP64 ? 1 : BC;
+  __shared__ HierQueue<TRY> queues[QW];
+  HierQueue<TRY>& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
   const int lane = threadIdx.x & 63;
   const double DS = p.ds, dsf = p.ds_fine, dm = p.dm;
   unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0, n_skip = 0;
+  unsigned own_fine = 0, own_skip = 0;  // TRY: fine counts of the ray's own steps
   double T = p.flat_field;
   Ray r;
   r.o[0] = r.o[1] = r.o[2] = 0.0;
@@ -609,11 +709,15 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
   bool prev_zero = true;  // main.go: prev_rho starts at 0
   int head = -1, tail = -1;  // this lane's queue entries, in coarse order
   int queued = 0;            // entries in the wave's queue (same on every lane)
-  // The coarse terms go into T and the fine sums, in coarse order, into
-  // Tf: the result, T + Tf, does not depend on when the queue is drained
-  // (nor, therefore, on which other rays share the wave or on how many
-  // samples the safe radius settles).
   double Tf = 0.0;
+  // TRY: lists a guard-band sample; a ray with too many stops
+  auto list = [&](uint32_t item) {
+    if (ts->cnt < p.defer_max) ts->ks[ts->cnt] = item;
+    if (++ts->cnt > p.defer_max) {
+      ts->aborted = true;
+      n = n_hi + 1;
+    }
+  };
   for (;;) {
     const int nb = n;
     unsigned tog = 0;
@@ -628,8 +732,9 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
       constexpr bool kDist = PREC == PREC_GUARDED;
       float dist = INFINITY;
       bool last_amb = true;
-      eval_along<PREC, B, kDist>(p, r.o, r.d, s, live, rho, n_reeval, n_miss, cl, &dist,
-                                 &last_amb);
+      unsigned amb = 0;
+      eval_along<PREC, B, kDist, TRY>(p, r.o, r.d, s, live, rho, n_reeval, n_miss, cl, amb,
+                                      &dist, &last_amb);
 #pragma unroll
       for (int q = 0; q < B; ++q) {
         if (!((live >> q) & 1u)) break;
@@ -645,6 +750,10 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
       }
       n_samples += (unsigned long long)__builtin_popcount(live);
       n = nb + B;
+      if constexpr (TRY) {
+        for (unsigned m = amb; m; m &= m - 1u)
+          list(((uint32_t)(nb + __builtin_ctz(m)) << 4) | 15u);
+      }
       // Coarse samples inside the last one's safe radius have its density:
       // no toggle, T += rho * DS each (main.go:189-191).
       if constexpr (kDist) {
@@ -652,12 +761,14 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
         if (live == (1u << B) - 1u && !last_amb && f >= 1.0f && n <= n_hi) {
           const int m = min(f < 65536.0f ? (int)f : 65536, n_hi - n + 1);
           const double rho_m = rho[B - 1];
-          if (p.audit) {
-            for (int j = 0; j < m; ++j) {
-              const double sj = p.s_table[n + j];
-              const double e = density64(p, r.o[0] + r.d[0] * sj, r.o[1] + r.d[1] * sj,
-                                         r.o[2] + r.d[2] * sj);
-              if (e != rho_m) ++n_miss;
+          if constexpr (!TRY) {
+            if (p.audit) {
+              for (int j = 0; j < m; ++j) {
+                const double sj = p.s_table[n + j];
+                const double e = density64(p, r.o[0] + r.d[0] * sj, r.o[1] + r.d[1] * sj,
+                                           r.o[2] + r.d[2] * sj);
+                if (e != rho_m) ++n_miss;
+              }
             }
           }
           const double c = (rho_m * dm) * DS;
@@ -704,51 +815,17 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
           d[c] = __shfl(r.d[c], src, 64);
         }
         if (has) {
-          // main.go:180-188: left = R[n-1] + ds, left += ds while left < right
-          const double right = p.s_table[nn];
-          double left = (nn == 0 ? p.smin : p.s_table[nn - 1]) + dsf;
-          double S = 0.0;
-          while (left < right) {
-            double s[B], rf[B];
-            unsigned live = 0;
-#pragma unroll
-            for (int q = 0; q < B; ++q) {
-              s[q] = left;
-              if (left < right) {
-                live |= 1u << q;
-                left += dsf;
-              }
-            }
-            constexpr bool kDist = PREC == PREC_GUARDED;
-            float dist = INFINITY;
-            bool last_amb = true;
-            eval_along<PREC, B, kDist>(p, o, d, s, live, rf, n_reeval, n_miss, cl, &dist,
-                                       &last_amb);
-#pragma unroll
-            for (int q = 0; q < B; ++q)
-              if ((live >> q) & 1u) S += (rf[q] * dm) * dsf;
-            n_fine += (unsigned long long)__builtin_popcount(live);
-            // Fine samples inside the last one's safe radius: same terms,
-            // same left += ds walk, no evaluation.
-            if constexpr (kDist) {
-              float f = dist * p.skip_inv * 10.0f;  // skip_inv is per DS = 10 ds
-              if (live == (1u << B) - 1u && !last_amb && f >= 1.0f) {
-                const double c = (rf[B - 1] * dm) * dsf;
-                for (int m = f < 64.0f ? (int)f : 64; m > 0 && left < right; --m) {
-                  if (p.audit) {
-                    const double e2 = density64(p, o[0] + d[0] * left, o[1] + d[1] * left,
-                                                o[2] + d[2] * left);
-                    if (e2 != rf[B - 1]) ++n_miss;
-                  }
-                  S += c;
-                  left += dsf;
-                  ++n_fine;
-                  ++n_skip;
-                }
-              }
-            }
+          unsigned nf, nsk, ambm;
+          Q.sum[e] = fine_walk<PREC, TRY, BF>(p, o, d, nn, cl, nf, nsk, ambm, n_reeval, n_miss);
+          if constexpr (TRY) {
+            // positions 0..14 fit the listing; 0xff (nsk < nf otherwise)
+            // marks a longer walk, whose ray stops
+            Q.ambm[e] = (uint16_t)ambm;
+            Q.cnt[e] = nf < 15u ? (uint8_t)(nf | (nsk << 4)) : (uint8_t)0xff;
+          } else {
+            n_fine += nf;
+            n_skip += nsk;
           }
-          Q.sum[e] = S;
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -756,6 +833,18 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       while (head >= 0) {
         Tf += Q.sum[head];
+        if constexpr (TRY) {
+          const unsigned c = Q.cnt[head];
+          if (c == 0xffu) {
+            ts->aborted = true;
+            n = n_hi + 1;
+          } else {
+            own_fine += c & 15u;
+            own_skip += c >> 4;
+            for (unsigned m = Q.ambm[head]; m; m &= m - 1u)
+              list(((uint32_t)Q.n[head] << 4) | (uint32_t)__builtin_ctz(m));
+          }
+        }
         head = Q.link[head];
       }
       tail = -1;
@@ -764,11 +853,17 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
     }
     if (!more) break;
   }
-  rc.samples += n_samples;
-  rc.reeval += n_reeval;
-  rc.miss += n_miss;
-  rc.fine += n_fine;
-  rc.skipped += n_skip;
+  if constexpr (TRY) {
+    ts->samples = (unsigned)n_samples;
+    ts->fine = own_fine;
+    ts->skipped = (unsigned)n_skip + own_skip;
+  } else {
+    rc.samples += n_samples;
+    rc.reeval += n_reeval;
+    rc.miss += n_miss;
+    rc.fine += n_fine;
+    rc.skipped += n_skip;
+  }
   return T + Tf;
 }
 
