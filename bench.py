@@ -455,10 +455,14 @@ def measured_counter(name: str, kernel: str, workload: str):
 C5_WORKLOAD = "gyroid6_tanh_256x1024"
 
 
-def march_kernel_name(st: dict) -> str:
-    """The guarded simple march's dominant kernel as the render ran it:
-    samples per node visit, and the first pass of the two-pass form (whose
-    resolve and redo launches are timed with it in kernel_ms)."""
+def march_kernel_name(st: dict, integrator: str = "simple") -> str:
+    """The guarded march's dominant kernel as the render ran it: samples per
+    node visit, and the first pass of the two-pass form (whose other
+    launches are timed with it in kernel_ms)."""
+    if integrator != "simple":
+        if st.get("two_pass"):
+            return f"march_hier_try_kernel<{st['block']},2>"
+        return "march_kernel<PREC_GUARDED,INT_HIERARCHICAL>"
     if st.get("two_pass"):
         return f"march_try_kernel<{st['block']}>"
     return f"march_kernel<PREC_GUARDED,INT_SIMPLE,{st['block']}>"
@@ -496,8 +500,7 @@ def bench_c5(args) -> int:
     value = proj / (ms_per_step / 1000.0)
     n_inbox = inbox_samples_device(w, lo, hi)
     k_ms = float(np.mean(kernel_ms))
-    kernel = (march_kernel_name(st) if integ == "simple"
-              else "march_kernel<PREC_GUARDED,INT_HIERARCHICAL>")
+    kernel = march_kernel_name(st, integ)
     achieved = FLOPS_PER_SAMPLE["gyroid"] * n_inbox / (k_ms / 1000.0) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
@@ -691,7 +694,7 @@ def main() -> int:
             "PREC_FP64" if prec == "fp64" else "PREC_GUARDED",
             "INT_SIMPLE" if w.integrator == "simple" else "INT_HIERARCHICAL")
         if st.get("block"):
-            roof["kernel"] = march_kernel_name(st)
+            roof["kernel"] = march_kernel_name(st, w.integrator)
     roof["skipped_samples"] = st.get("skipped_samples", 0)
     if roof["bound"] == "hbm":
         # the gathers mostly hit L2: also state them against the measured L2

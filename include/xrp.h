@@ -125,7 +125,9 @@ typedef struct {
   double position_error;   /* FP32 object-space position error bound used */
   int32_t precision_used;  /* xrp_precision actually run */
   int32_t block;           /* samples per node visit of the guarded simple march
-                              (1, 2 or 4); 0 for the other kernels */
+                              (1, 2 or 4), or coarse samples per node visit of the
+                              two-pass hierarchical march's first pass; 0 for the
+                              other kernels */
   uint64_t staged_chunks;   /* voxel kernel: sample chunks served from LDS */
   uint64_t staged_voxels;   /* voxel kernel: voxels copied into LDS */
   uint64_t fallback_chunks; /* voxel kernel: chunks gathered from global memory */
@@ -135,9 +137,11 @@ typedef struct {
                                evaluated sample before them; under audit each is
                                also evaluated in FP64 and compared */
   uint64_t redo_rays;       /* two-pass march: rays marched again (more guard-band
-                               samples than a deferral slot holds, or no slot free) */
+                               samples than a deferral slot holds, no slot free, or,
+                               hierarchical, an FP32 guess that was not exact) */
   uint64_t deferred_rays;   /* two-pass march: rays whose guard-band samples were
-                               re-evaluated in FP64 after the march */
+                               re-evaluated (hierarchical: checked) in FP64 after
+                               the march */
   int32_t two_pass;         /* 1 when the guarded simple march ran in two passes: the
                                march without FP64, guard-band samples re-evaluated after */
   int32_t reserved1;

@@ -155,9 +155,8 @@ def test_two_pass_by_job_size(ctx, monkeypatch):
 
 
 def test_two_pass_only_where_it_applies(ctx, monkeypatch):
-    """Audit, FP64 and, for the simple integrator, 4-sample blocks (24 nodes
-    and up) keep the one-pass kernels; the guarded hierarchical march takes
-    two passes at any block size."""
+    """Audit, FP64 and 4-sample blocks (24 nodes and up) keep the one-pass
+    kernels, for either integrator."""
     monkeypatch.setenv("XRP_TWO_PASS", "1")
     w = W.csg(resolution=48, n_views=1)
     sc = xrp_scene(w.obj)
@@ -175,7 +174,7 @@ def test_two_pass_only_where_it_applies(ctx, monkeypatch):
     _, st = X.render(ctx, sc, w.angles, 48, sc.auto_ds())
     assert st["block"] == 4 and st["two_pass"] == 0
     _, st = X.render(ctx, sc, w.angles, 48, sc.auto_ds(), integrator="hierarchical")
-    assert st["two_pass"] == 1
+    assert st["two_pass"] == 0
     sc.close()
 
 

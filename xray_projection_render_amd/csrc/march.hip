@@ -181,7 +181,7 @@ march_try_kernel(MarchParams p) {
 // such samples (at most p.defer_max <= kHierDeferMax) list them for
 // march_hier_check_kernel.  Stopped rays go to march_hier_redo_kernel.
 template <int BC, int BF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BC == 4 ? 2 : 3))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void
 march_hier_try_kernel(MarchParams p) {
   __shared__ uint32_t kbuf[64 * kHierDeferMax];
   const int lane = threadIdx.x;
@@ -562,14 +562,11 @@ int launch_march(const MarchParams& p, int precision, int integrator,
       const dim3 grid3(std::min((p.defer_cap + 63u) / 64u, 8192u));
       const dim3 grid2(std::min((p.redo_cap + 63u) / 64u, 3072u));
       // coarse samples per node visit as the simple march's block (1 where
-      // gyroids reach far, 4 on large collections), 2 per fine visit on
-      // small trees
+      // gyroids reach far, else 2), 2 per fine visit
       if (p.block == 1)
         hipLaunchKernelGGL((march_hier_try_kernel<1, 2>), grid, block, 0, s, p);
-      else if (p.block == 2)
-        hipLaunchKernelGGL((march_hier_try_kernel<2, 2>), grid, block, 0, s, p);
       else
-        hipLaunchKernelGGL((march_hier_try_kernel<4, 4>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((march_hier_try_kernel<2, 2>), grid, block, 0, s, p);
       hipLaunchKernelGGL(march_hier_check_kernel, grid3, block, 0, s, p);
       hipLaunchKernelGGL(march_hier_redo_kernel, grid2, block, 0, s, p);
     } else if (integrator == INT_HIERARCHICAL)
