@@ -7,8 +7,10 @@ rho 1, under z += 0.1 tanh(8x); 1024x1024 pixels, 256 views at azimuth
 k*360/256, polar 90, simple integrator, ds 0.001.  The plain form (guarded
 FP32 march, exact against the FP64 oracle) gives `value`; the surrogate form
 (hash-grid MLP trained on 2^24 exact samples, 16x16 tiles checked against
-the exact march and re-rendered above 1e-3) is timed beside it.  Other
-workloads: --workload voxel|csg|lattice|gyroid|batch|surrogate.
+the exact march and re-rendered above 1e-3) is timed beside it, and so are
+the same views under the hierarchical integrator, the reference's default
+(`hierarchical` object).  Other workloads: --workload
+voxel|csg|lattice|gyroid|batch|surrogate.
 
 Prints one JSON line with the throughput, the dominant kernel's roofline
 position (HIP-event kernel time, algorithmic bytes or flops per sample)
@@ -526,6 +528,8 @@ def bench_c5(args) -> int:
         "reevaluated": st["reevaluated"],
         "roofline": roof,
     }
+    if integ == "simple" and not args.no_hier:
+        line["hierarchical"] = c5_hier_leg(args, ctx, sc, w, n_inbox, skip_cpu=args.no_cpu)
     if not args.no_surrogate:
         line["surrogate"] = c5_surrogate_leg(args, ctx, sc, w, exact, kw, ms_per_step)
     if not args.no_cpu:
@@ -540,6 +544,49 @@ def bench_c5(args) -> int:
     sc.close()
     ctx.close()
     return 0
+
+
+def c5_hier_leg(args, ctx, sc, w, n_inbox: int, skip_cpu: bool) -> dict:
+    """The same views under the hierarchical integrator, the reference's
+    default (main.go:40, api.go:128-132): 1 warm-up and up to 3 timed
+    renders, and (with the CPU leg) its error against the FP64 oracle on a
+    band of view 0."""
+    import torch
+    res = w.resolution
+    img = torch.empty((len(w.angles), res, res), dtype=torch.float32, device="cuda:0")
+    kw = dict(integrator="hierarchical", precision=args.precision, R=w.R, fov=w.fov)
+    st = X.render_device(ctx, sc, w.angles, res, w.ds, img.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 3))
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = X.render_device(ctx, sc, w.angles, res, w.ds, img.data_ptr(), **kw)
+        kernel_ms.append(st["kernel_ms"])
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    print(f"c5: hierarchical {steps} steps, {ms:.1f} ms each", file=sys.stderr, flush=True)
+    out = {"integrator": "hierarchical", "steps": steps, "ms_per_step": ms,
+           "value": w.pixels / (512.0 * 512.0) / (ms / 1000.0), "unit": "512² projections/s",
+           "kernel": march_kernel_name(st, "hierarchical"),
+           "kernel_ms": float(np.mean(kernel_ms)), "coarse_samples": n_inbox,
+           "fine_samples": st["fine_samples"], "two_pass": st["two_pass"],
+           "checked_rays": st["deferred_rays"], "redo_rays": st["redo_rays"],
+           "traffic": measured_traffic(march_kernel_name(st, "hierarchical"), C5_WORKLOAD)}
+    if not skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        osc = O.Scene(O.build_object(w.obj), O.build_deformation(w.deformation))
+        r0, rows = res // 2 - 2, 4
+        az, po = w.angles[0]
+        ref = osc.render([az], [po], res, w.ds, R=w.R, fov=w.fov, integrator="hierarchical",
+                         nthreads=os.cpu_count() or 1, rows=(r0, r0 + rows))[0, r0:r0 + rows]
+        got = img[0, r0:r0 + rows].double().cpu().numpy()
+        out["error_vs_oracle"] = {"max_abs": float(np.max(np.abs(got - ref))),
+                                  "pixels": int(ref.size), "tolerance": 1e-5,
+                                  "sample": f"view 0 rows {r0}..{r0 + rows - 1}"}
+    del img
+    return out
 
 
 def c5_surrogate_leg(args, ctx, sc, w, exact, kw, plain_ms: float) -> dict:
@@ -604,6 +651,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c5")
+    ap.add_argument("--no-hier", action="store_true",
+                    help="c5: skip the hierarchical-integrator leg")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="c5: time the plain form only")
     ap.add_argument("--surrogate-steps", type=int, default=1,
