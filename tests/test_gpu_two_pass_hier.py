@@ -104,6 +104,34 @@ def test_hier_two_pass_matches_one_pass_and_oracle(ctx, monkeypatch, kind, arg):
     assert float(np.max(np.abs(a.astype(np.float64) - ref))) <= TOL
 
 
+_AFFINE = {"type": "affine",
+           "matrix": [[1.2, 0.15, -0.1], [-0.2, 0.9, 0.25], [0.1, -0.15, 1.1]]}
+
+
+@pytest.mark.parametrize("seed", [0, 2, 4, 6, 14])
+def test_hier_two_pass_culled_collections(ctx, monkeypatch, seed):
+    """Root collections of 8-20 children, culled per wave, under an affine
+    map: the first pass guesses with the wave's cull list, the check pass
+    computes the guesses again without one (the padded cull spheres keep
+    culled children clear of every guard band), and the images keep the
+    one-pass march's bits."""
+    obj, _, _ = W.random_scene(seed, max_prims=20)
+    assert 8 <= len(obj["objects"]) <= 20
+    sc = xrp_scene(obj, _AFFINE)
+    ds = sc.auto_ds()
+    angles = [(20.0, 80.0), (110.0, 95.0), (250.0, 60.0)]
+    a, sa, b, sb = _pair(ctx, monkeypatch, sc, angles, 80, ds)
+    sc.close()
+    print(f"seed {seed}: checked {sa['deferred_rays']} redo {sa['redo_rays']} of "
+          f"{sa['rays']}", flush=True)
+    assert sa["two_pass"] == 1
+    _same(a, b)
+    assert sa["samples"] == sb["samples"] and sa["fine_samples"] == sb["fine_samples"]
+    osc = oracle_scene(obj, _AFFINE)
+    ref = oracle_render(osc, angles, 80, ds, integrator="hierarchical")
+    assert float(np.max(np.abs(a.astype(np.float64) - ref))) <= TOL
+
+
 @pytest.mark.parametrize("most", ["0", "1"])
 def test_hier_two_pass_marches_again_what_does_not_fit(ctx, monkeypatch, most):
     """A ray's slots cut to hold at most 0 or 1 listed samples
