@@ -143,13 +143,21 @@ def test_two_pass_gray8_output(ctx, monkeypatch):
 
 def test_two_pass_by_job_size(ctx, monkeypatch):
     """Without the switch, jobs under 2^21 rays take one pass and larger
-    ones two."""
+    ones two; the hierarchical march keeps one pass for a root collection
+    and takes two for a tessellation."""
     monkeypatch.delenv("XRP_TWO_PASS", raising=False)
     w = W.csg(resolution=1024, n_views=2)
     sc = xrp_scene(w.obj)
     _, st = X.render(ctx, sc, w.angles[:1], 1024, w.ds)
     assert st["two_pass"] == 0
     _, st = X.render(ctx, sc, w.angles, 1024, w.ds)
+    assert st["two_pass"] == 1
+    _, st = X.render(ctx, sc, w.angles, 1024, 10 * w.ds, integrator="hierarchical")
+    assert st["two_pass"] == 0
+    sc.close()
+    g = W.gyroid_lattice(resolution=1024, n_views=2, ds=0.01, shear=True)
+    sc = xrp_scene(g.obj, g.deformation)
+    _, st = X.render(ctx, sc, g.angles, 1024, g.ds, integrator="hierarchical")
     assert st["two_pass"] == 1
     sc.close()
 

@@ -862,7 +862,12 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   // in two passes (march.hip march_try_kernel, march_hier_try_kernel), for
   // jobs of 2^21 rays and up.  On large collections a ray meets more
   // guard-band samples than it keeps (179 primitives, hierarchical: 76 % of
-  // the rays marched again, 497 against 336 ms in one pass) (below, the two extra
+  // the rays marched again, 497 against 336 ms in one pass).  The
+  // hierarchical form is not taken for root collections (XRP_TWO_PASS=1
+  // takes it, for tests): boxes and other faces that views cross edge-on
+  // put samples on the faces, whose FP32 guesses often fail (random
+  // collections of 8-20 children, 16 x 512^2: 3.6-12.5 ms in one pass,
+  // 26-42 % slower in two) (below, the two extra
   // launches cost more than they save: CSG 8 x 256^2 0.107 against 0.097
   // ms).  A launch covers at most 2^31 rays, which the lists name by 32-bit
   // index.  Not under audit, nor without allocation.
@@ -870,7 +875,8 @@ int render_core(xrp_context* ctx, xrp_scene* sc, const RenderJob& job,
   const bool hier = job.integrator == XRP_INTEGRATE_HIERARCHICAL;
   const bool two_pass = !fixed && !one_pass && !sc->one_pass &&
                         precision == XRP_PRECISION_GUARDED && !vox_kernel && !job.audit &&
-                        (hier || !pl.mp.queue) && (pl.mp.block == 1 || pl.mp.block == 2) &&
+                        (hier ? s.root.kind != K_COLL || two_pass_forced() : !pl.mp.queue) &&
+                        (pl.mp.block == 1 || pl.mp.block == 2) &&
                         ((size_t)nv * res * res >= ((size_t)1 << 21) || two_pass_forced()) &&
                         two_pass_enabled();
   int vpc = two_pass ? (int)std::min<size_t>(kMaxGridY, ((size_t)1 << 31) / ((size_t)res * res))
