@@ -182,3 +182,29 @@ def test_hier_two_pass_split_launches(ctx, monkeypatch):
     assert sa["two_pass"] == 1
     _same(a, b)
     assert sa["samples"] == sb["samples"] and sa["fine_samples"] == sb["fine_samples"]
+
+
+@pytest.mark.parametrize("kw", [dict(output="gray8_alpha8", transparency=True),
+                                dict(flat_field=0.3, density_multiplier=1.7, R=3.5, fov=50.0)],
+                         ids=["gray8_alpha8", "flat_field_dm"])
+def test_hier_two_pass_outputs_and_parameters(ctx, monkeypatch, kw):
+    """Byte output with transparency, and a flat field, density multiplier,
+    camera distance and field of view other than the defaults: the same
+    bytes or bits as one pass, and (float output) the oracle's image."""
+    w = W.gyroid_lattice(resolution=72, n_views=3, ds=0.002, shear=True)
+    sc = xrp_scene(w.obj, w.deformation)
+    monkeypatch.setenv("XRP_TWO_PASS", "1")
+    a, sa = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="hierarchical", **kw)
+    monkeypatch.setenv("XRP_TWO_PASS", "0")
+    b, sb = X.render(ctx, sc, w.angles, w.resolution, w.ds, integrator="hierarchical", **kw)
+    sc.close()
+    print(f"{kw}: checked {sa['deferred_rays']} redo {sa['redo_rays']}", flush=True)
+    assert sa["two_pass"] == 1 and sb["two_pass"] == 0
+    assert np.array_equal(a, b)
+    if "flat_field" in kw:
+        osc = oracle_scene(w.obj, w.deformation, density_multiplier=kw["density_multiplier"],
+                           flat_field=kw["flat_field"])
+        ref = oracle_render(osc, w.angles, w.resolution, w.ds, integrator="hierarchical",
+                            R=kw["R"], fov=kw["fov"])
+        assert float(np.max(np.abs(a.astype(np.float64) - ref))) <= TOL
+
