@@ -154,8 +154,11 @@ __device__ __forceinline__ void try_finish(const MarchParams& p, int v, int i, i
 // sample indices, to the resolve pass's lists; one with more, or finding
 // the lists full, to p.redo_list for march_redo_kernel.  Without the FP64
 // path in the loop no lane waits on another's FP64 re-evaluation.
+// 6 waves per SIMD at one sample per visit (80 VGPRs, some spilled; 7 and 8
+// waves spill more and ran 21 % and 18 % slower), 5 at two: on C5 5 -> 6
+// waves took 57.5 to 54.5 ms, on the lattice 4 -> 5 took 39.3 to 35.4 ms.
 template <int B>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(B == 1 ? 6 : 5))) void
 march_try_kernel(MarchParams p) {
   __shared__ uint32_t kbuf[64 * kDeferMax];
   const int lane = threadIdx.x;

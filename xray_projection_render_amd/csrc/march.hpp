@@ -126,10 +126,11 @@ struct MarchParams {
 };
 
 // Guard-band samples a ray may defer to the resolve pass (the first pass
-// keeps them in LDS: 8 KiB per wave).  A ray with more is marched again,
-// by one wave with few rays, which takes long: on the sheared gyroid
-// lattice 8 left 78 rays of 33.5 M to the redo pass (0.9 ms).
-constexpr uint32_t kDeferMax = 32;
+// keeps them in LDS: 6 KiB per wave, so that 24 one-wave workgroups, 6 per
+// SIMD, fit a CU).  A ray with more is marched again, by one wave with few
+// rays, which takes long: on the sheared gyroid lattice 32 slots left 78
+// rays of 33.5 M to the redo pass (0.9 ms).
+constexpr uint32_t kDeferMax = 24;
 // Guard-band samples a ray the first pass of the two-pass hierarchical
 // march may list (LDS: 3.5 KiB per wave beside the toggle queue and the
 // cull list, so that 12 one-wave workgroups fit a CU's 160 KiB).
