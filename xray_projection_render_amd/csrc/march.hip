@@ -183,8 +183,12 @@ march_try_kernel(MarchParams p) {
 // FP32 values; every ray not stopped stores its pixel, and the rays with
 // such samples (at most p.defer_max <= kHierDeferMax) list them for
 // march_hier_check_kernel.  Stopped rays go to march_hier_redo_kernel.
+// At 2 coarse samples per visit 4 waves per SIMD, with a 320-entry toggle
+// queue so that 16 workgroups fit a CU's LDS: lattice 84.0 -> 80.2 ms.  At
+// 1, where the extra wave spills in the march loop, 3 waves and 512
+// entries (C5: 176.5 against 191.6 ms at 4 waves).
 template <int BC, int BF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BC == 2 ? 4 : 3))) void
 march_hier_try_kernel(MarchParams p) {
   __shared__ uint32_t kbuf[64 * kHierDeferMax];
   const int lane = threadIdx.x;
@@ -200,7 +204,8 @@ march_hier_try_kernel(MarchParams p) {
   const Cull cl = wave_cull<1>(p, v, i, j, valid);
   TryState ts;
   ts.ks = kbuf + lane * kHierDeferMax;
-  const double T = march_T_hier<PREC_GUARDED, 1, true, BC, BF>(p, v, i, j, valid, rc, cl, &ts);
+  const double T = march_T_hier<PREC_GUARDED, 1, true, BC, BF, BC == 2 ? 320 : kHierCap>(
+      p, v, i, j, valid, rc, cl, &ts);
   try_finish<true>(p, v, i, j, valid, T, ts, rc);
 }
 

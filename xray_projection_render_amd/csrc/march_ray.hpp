@@ -506,16 +506,17 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
 
 constexpr int kHierCap = 512;   // queued toggles per wave
 
-// Per-wave toggle queue.  TRY (the first pass of the two-pass march) also
-// keeps each walk's guard-band positions and counts for the owning lane.
-template <bool TRY>
+// Per-wave toggle queue of CAP entries.  TRY (the first pass of the
+// two-pass march) also keeps each walk's guard-band positions and counts
+// for the owning lane.
+template <bool TRY, int CAP = kHierCap>
 struct HierQueue {
-  double sum[kHierCap];               // fine sum of the step: sum of (rho * dm) * DS/10
-  int32_t n[kHierCap];                // coarse index of the step's right endpoint
-  int16_t link[kHierCap];             // the owning lane's next entry, -1 for none
-  uint16_t ambm[TRY ? kHierCap : 1];  // bit m: the walk's m-th sample was in a guard band
-  uint8_t cnt[TRY ? kHierCap : 1];    // the walk's fine samples | settled ones << 4
-  uint8_t owner[kHierCap];            // lane whose ray the step belongs to
+  double sum[CAP];               // fine sum of the step: sum of (rho * dm) * DS/10
+  int32_t n[CAP];                // coarse index of the step's right endpoint
+  int16_t link[CAP];             // the owning lane's next entry, -1 for none
+  uint16_t ambm[TRY ? CAP : 1];  // bit m: the walk's m-th sample was in a guard band
+  uint8_t cnt[TRY ? CAP : 1];    // the walk's fine samples | settled ones << 4
+  uint8_t owner[CAP];            // lane whose ray the step belongs to
 };
 
 // Densities (dm not applied) at s[q] along the ray o + s d, for the samples
@@ -676,13 +677,16 @@ __device__ __forceinline__ double fine_walk(const MarchParams& p, const double (
 //
 // BC, BF: coarse and fine samples per node visit.  Every term is the exact
 // density of its sample added in order, so the sums do not depend on them.
-template <int PREC, int QW, bool TRY = false, int BC = kHierBlock, int BF = kHierBlock>
+// CAP: toggle queue entries (drained once more than CAP - 64 BC wait).
+template <int PREC, int QW, bool TRY = false, int BC = kHierBlock, int BF = kHierBlock,
+          int CAP = kHierCap>
 __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int i, int j,
                                                bool valid, RayCounts& rc, const Cull& cl,
                                                TryState* ts = nullptr) {
   constexpr int B = PREC == PREC_FP64 ? 1 : BC;
-  __shared__ HierQueue<TRY> queues[QW];
-  HierQueue<TRY>& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
+  static_assert(CAP >= 64 * (PREC == PREC_FP64 ? 1 : BC) + 64, "queue too small");
+  __shared__ HierQueue<TRY, CAP> queues[QW];
+  HierQueue<TRY, CAP>& Q = queues[QW == 1 ? 0 : (threadIdx.x >> 6)];
   const int lane = threadIdx.x & 63;
   const double DS = p.ds, dsf = p.ds_fine, dm = p.dm;
   unsigned long long n_samples = 0, n_reeval = 0, n_miss = 0, n_fine = 0, n_skip = 0;
@@ -799,7 +803,7 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
       queued += (int)__popcll(m);
     }
     const bool more = __ballot(n <= n_hi) != 0ull;
-    if (queued > kHierCap - 64 * B || (!more && queued > 0)) {
+    if (queued > CAP - 64 * B || (!more && queued > 0)) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
