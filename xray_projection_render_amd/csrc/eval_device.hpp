@@ -259,6 +259,43 @@ __device__ __forceinline__ double density64(const MarchParams& p, double x,
   return node64<0>(p.nodes, 0, x, y, z);
 }
 
+// density64 with the root collection's children limited to idx[0 .. n)
+// (increasing node indices, the same for every lane of the wave; n < 0:
+// all of them).  The list is the wave's cull list: every child left out
+// has density 0 along the wave's rays, so the sum over the list (in the
+// same order), and the result, are those of the full walk
+// (objects.go:422-438).
+__device__ __forceinline__ double density64(const MarchParams& p, double x, double y,
+                                            double z, const int16_t* idx, int n) {
+  deform64(p.defs, p.n_defs, x, y, z);
+  if (n < 0) return node64<0>(p.nodes, 0, x, y, z);
+  const DevNode* __restrict__ N = p.nodes;
+  const bool greedy = N[0].greedy;
+  double density = 0.0;
+  for (int t = 0; t < n; ++t) {
+    const int c = __builtin_amdgcn_readfirstlane((int)idx[t]);
+    const DevNode& k = N[c];
+    double rho;
+    switch (k.kind) {
+      case K_SPHERE: rho = sphere64(k, x, y, z); break;
+      case K_BOX: rho = box64(k, x, y, z); break;
+      case K_PPED: rho = pped64(k, x, y, z); break;
+      case K_CYL: rho = cyl64(k, x, y, z); break;
+      case K_GYROID: rho = gyroid64(k, x, y, z); break;
+      case K_VOXEL: rho = voxel64(k, x, y, z); break;
+      case K_TESS: rho = tess64<0>(N, c, x, y, z); break;
+      default: rho = 0.0;
+    }
+    if (greedy && rho > 0.0) return rho;
+    density += rho;
+  }
+  if (density < 0.0)
+    density = 0.0;
+  else if (density > 1.0)
+    density = 1.0;
+  return density;
+}
+
 
 // ---------------------------------------------------------------------------
 // Guarded FP32 evaluator.  Returns the exact FP64 density when every

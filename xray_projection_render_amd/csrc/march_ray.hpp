@@ -229,7 +229,7 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
           if (q == t) vq = vv[t];
         const double s = p.s_table[k + q];
         const double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                   r.o[2] + r.d[2] * s);
+                                   r.o[2] + r.d[2] * s, cl.idx, cl.n);
         if ((amb >> q) & 1u) {
           ++n_reeval;
           Ta += (e * dm) * ds;
@@ -255,7 +255,7 @@ __device__ __forceinline__ double march_T_guarded_inline(const MarchParams& p, i
             for (int jj = 0; jj < m; ++jj) {
               double s = p.s_table[k + jj];
               double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                   r.o[2] + r.d[2] * s);
+                                   r.o[2] + r.d[2] * s, cl.idx, cl.n);
               if (e != rho_m) ++n_miss;
             }
           }
@@ -387,7 +387,7 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
             if (q == t) vq = vv[t];
           const double s = p.s_table[k + q];
           const double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                     r.o[2] + r.d[2] * s);
+                                     r.o[2] + r.d[2] * s, cl.idx, cl.n);
           if ((amb >> q) & 1u)
             Ta += (e * dm) * ds;
           else if (e != vq)
@@ -414,7 +414,7 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
             for (int jj = 0; jj < m; ++jj) {
               double s = p.s_table[k + jj];
               double e = density64(p, r.o[0] + r.d[0] * s, r.o[1] + r.d[1] * s,
-                                   r.o[2] + r.d[2] * s);
+                                   r.o[2] + r.d[2] * s, cl.idx, cl.n);
               if (e != rho_m) ++n_miss;
             }
           }
@@ -465,7 +465,7 @@ __device__ __forceinline__ double march_T_guarded(const MarchParams& p, int v, i
         }
         if (has) {
           const double s = p.s_table[kk];
-          Q.val[e] = density64(p, o[0] + d[0] * s, o[1] + d[1] * s, o[2] + d[2] * s);
+          Q.val[e] = density64(p, o[0] + d[0] * s, o[1] + d[1] * s, o[2] + d[2] * s, cl.idx, cl.n);
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -542,7 +542,7 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
 #pragma unroll
     for (int q = 0; q < B; ++q)
       rho[q] = ((live >> q) & 1u)
-                   ? density64(p, o[0] + d[0] * s[q], o[1] + d[1] * s[q], o[2] + d[2] * s[q])
+                   ? density64(p, o[0] + d[0] * s[q], o[1] + d[1] * s[q], o[2] + d[2] * s[q], cl.idx, cl.n)
                    : 0.0;
   } else {
     PBlock<B> P;
@@ -576,7 +576,7 @@ __device__ __forceinline__ void eval_along(const MarchParams& p, const double (&
             sq = s[t];
             rq = rho[t];
           }
-        const double e = density64(p, o[0] + d[0] * sq, o[1] + d[1] * sq, o[2] + d[2] * sq);
+        const double e = density64(p, o[0] + d[0] * sq, o[1] + d[1] * sq, o[2] + d[2] * sq, cl.idx, cl.n);
         if ((amb >> q) & 1u)
           ++n_reeval;
         else if (e != rq)
@@ -642,7 +642,7 @@ __device__ __forceinline__ double fine_walk(const MarchParams& p, const double (
           if constexpr (!TRY) {
             if (p.audit) {
               const double e2 = density64(p, o[0] + d[0] * left, o[1] + d[1] * left,
-                                          o[2] + d[2] * left);
+                                          o[2] + d[2] * left, cl.idx, cl.n);
               if (e2 != rf[B - 1]) ++n_miss;
             }
           }
@@ -770,7 +770,7 @@ __device__ __forceinline__ double march_T_hier(const MarchParams& p, int v, int 
               for (int j = 0; j < m; ++j) {
                 const double sj = p.s_table[n + j];
                 const double e = density64(p, r.o[0] + r.d[0] * sj, r.o[1] + r.d[1] * sj,
-                                           r.o[2] + r.d[2] * sj);
+                                           r.o[2] + r.d[2] * sj, cl.idx, cl.n);
                 if (e != rho_m) ++n_miss;
               }
             }
