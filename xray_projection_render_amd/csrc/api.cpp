@@ -765,8 +765,8 @@ int plan_job(xrp_context* ctx, xrp_scene* sc, const RenderJob& job, LaunchPlan& 
       std::memcpy(mp.M, cM, sizeof cM);
       std::memcpy(mp.t, ct, sizeof ct);
       // a culled child stays further than the guard band from every FP32
-      // sample position, so it changes no FP32 density (the two-pass
-      // hierarchical march evaluates listed samples again without culling);
+      // sample position, so it changes no FP32 density (the check pass of
+      // the two-pass hierarchical march culls again for its own waves);
       // its FP64 density along the wave's rays is 0, so the FP64
       // re-evaluations walk the same list
       const double pad = 2.0 * delta * (1.0 + 1e-6);

@@ -216,7 +216,8 @@ march_hier_try_kernel(MarchParams p) {
 // which the first pass used, is computed again (it depends on the position
 // alone) and compared with its FP64 density; a ray with any that differs
 // goes to the redo list, whose pass overwrites its pixel, and its first
-// pass counts (after its items) are taken back from the stats.
+// pass counts (after its items) are taken back from the stats.  Each wave
+// culls the root collection's children to its own 64 rays first.
 // 4 waves per SIMD (128 VGPRs, the FP64 evaluator's spilled): on C5 24.5 ms
 // at 2 waves, 18.5 ms at 4, 22.2 ms at 5.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
@@ -224,14 +225,22 @@ march_hier_check_kernel(MarchParams p) {
   const unsigned n = min(p.counts[1], p.defer_cap), nk = min(p.counts[2], p.k_cap);
   const unsigned rr = (unsigned)p.res * (unsigned)p.res;
   const double dsf = p.ds_fine;
-  const Cull none;
   unsigned long long reeval = 0, rays = 0, back_s = 0, back_f = 0, back_k = 0;
-  for (unsigned e = blockIdx.x * 64u + threadIdx.x; e < n; e += gridDim.x * 64u) {
-    const unsigned cnt = p.defer_cnt[e], k0 = p.defer_kfirst[e];
-    if (cnt == 0u || k0 + cnt + 3u > nk) continue;
-    const unsigned id = p.defer_ray[e];
+  for (unsigned base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+    const unsigned e = base + threadIdx.x;
+    unsigned cnt = 0u, k0 = 0u, id = 0u;
+    if (e < n) {
+      cnt = p.defer_cnt[e];
+      k0 = p.defer_kfirst[e];
+      id = p.defer_ray[e];
+      if (k0 + cnt + 3u > nk) cnt = 0u;
+    }
     const unsigned v = id / rr, pix = id - v * rr;
     const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
+    // this wave's own cull list over its rays (a child culled here is clear
+    // of every one of them in FP32 and 0 in FP64, as in the first pass)
+    const Cull cl = wave_cull<1>(p, (int)v, i, j, cnt != 0u);
+    if (cnt == 0u) continue;
     const Ray r = make_ray(p, p.views[p.view_offset + (int)v], i, j);
     bool same = true;
     for (unsigned c = 0; c < cnt && same; ++c) {
@@ -246,10 +255,10 @@ march_hier_check_kernel(MarchParams p) {
       double f32[1];
       unsigned amb;
       unsigned long long unused = 0;
-      eval_along<PREC_GUARDED, 1, false, true>(p, r.o, r.d, s, 1u, f32, unused, unused, none,
+      eval_along<PREC_GUARDED, 1, false, true>(p, r.o, r.d, s, 1u, f32, unused, unused, cl,
                                                amb);
       const double ev = density64(p, r.o[0] + r.d[0] * s[0], r.o[1] + r.d[1] * s[0],
-                                  r.o[2] + r.d[2] * s[0]);
+                                  r.o[2] + r.d[2] * s[0], cl.idx, cl.n);
       same = ev == f32[0];
     }
     if (same) {
