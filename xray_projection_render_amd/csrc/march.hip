@@ -209,17 +209,20 @@ march_hier_try_kernel(MarchParams p) {
   try_finish<true>(p, v, i, j, valid, T, ts, rc);
 }
 
-// Check pass of the two-pass hierarchical march: one lane per entry
-// (grid-stride).  Each listed sample, (nn << 4) | m, is at R[nn] for m = 15
-// and otherwise at R[nn-1] + DS/10 plus m more additions of DS/10 (the
+// Check pass of the two-pass hierarchical march: 64 entries per wave
+// (grid-stride), their listed samples spread one per lane.  Each listed
+// sample, (nn << 4) | m, is at R[nn] for m = 15 and otherwise at
+// R[nn-1] + DS/10 plus m more additions of DS/10 (the
 // m-th sample of step nn's fine walk, main.go:180-186).  Its FP32 value,
 // which the first pass used, is computed again (it depends on the position
 // alone) and compared with its FP64 density; a ray with any that differs
 // goes to the redo list, whose pass overwrites its pixel, and its first
 // pass counts (after its items) are taken back from the stats.  Each wave
 // culls the root collection's children to its own 64 rays first.
-// 4 waves per SIMD (128 VGPRs, the FP64 evaluator's spilled): on C5 24.5 ms
-// at 2 waves, 18.5 ms at 4, 22.2 ms at 5.
+// One listed sample per lane rather than one ray per lane: on C5 18.6 ->
+// 11.3 ms (a wave ran as many rounds as its longest list, 1-14 samples).
+// 4 waves per SIMD (128 VGPRs, the FP64 evaluator's spilled): 24.5 ms at 2
+// waves, 18.5 ms at 4, 22.2 ms at 5, one ray per lane.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
 march_hier_check_kernel(MarchParams p) {
   const unsigned n = min(p.counts[1], p.defer_cap), nk = min(p.counts[2], p.k_cap);
@@ -232,35 +235,71 @@ march_hier_check_kernel(MarchParams p) {
     if (e < n) {
       cnt = p.defer_cnt[e];
       k0 = p.defer_kfirst[e];
-      id = p.defer_ray[e];
       if (k0 + cnt + 3u > nk) cnt = 0u;
+      // an entry marked empty (cnt 0, lists full) has no ray or k0 written
+      if (cnt != 0u)
+        id = p.defer_ray[e];
+      else
+        k0 = 0u;
     }
     const unsigned v = id / rr, pix = id - v * rr;
     const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
     // this wave's own cull list over its rays (a child culled here is clear
     // of every one of them in FP32 and 0 in FP64, as in the first pass)
     const Cull cl = wave_cull<1>(p, (int)v, i, j, cnt != 0u);
-    if (cnt == 0u) continue;
     const Ray r = make_ray(p, p.views[p.view_offset + (int)v], i, j);
-    bool same = true;
-    for (unsigned c = 0; c < cnt && same; ++c) {
-      const unsigned item = p.defer_k[k0 + c], nn = item >> 4, m = item & 15u;
-      double s[1];
-      if (m == 15u) {
-        s[0] = p.s_table[nn];
-      } else {
-        s[0] = (nn == 0 ? p.smin : p.s_table[nn - 1]) + dsf;
-        for (unsigned q = 0; q < m; ++q) s[0] += dsf;
-      }
-      double f32[1];
-      unsigned amb;
-      unsigned long long unused = 0;
-      eval_along<PREC_GUARDED, 1, false, true>(p, r.o, r.d, s, 1u, f32, unused, unused, cl,
-                                               amb);
-      const double ev = density64(p, r.o[0] + r.d[0] * s[0], r.o[1] + r.d[1] * s[0],
-                                  r.o[2] + r.d[2] * s[0], cl.idx, cl.n);
-      same = ev == f32[0];
+    // The wave's listed samples, one per lane: lane l of round fb takes
+    // sample t = fb + l of the concatenated lists, owned by the ray (lane o)
+    // whose inclusive prefix sum of counts is the first above t.
+    unsigned incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned u = __shfl_up(incl, off, 64);
+      if ((int)threadIdx.x >= off) incl += u;
     }
+    const unsigned total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    unsigned long long bad = 0ull;  // lanes whose ray holds a wrong guess
+    for (unsigned fb = 0; fb < total; fb += 64u) {
+      const unsigned t = fb + threadIdx.x;
+      int o = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1)
+        if (__shfl(incl, o + step - 1, 64) <= t) o += step;
+      const bool has = t < total;
+      const unsigned c = t - (__shfl(incl, o, 64) - __shfl(cnt, o, 64));
+      const unsigned ko = __shfl(k0, o, 64);
+      double ro[3], rd[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        ro[q] = __shfl(r.o[q], o, 64);
+        rd[q] = __shfl(r.d[q], o, 64);
+      }
+      bool wrong = false;
+      if (has) {
+        const unsigned item = p.defer_k[ko + c], nn = item >> 4, m = item & 15u;
+        double sq[1];
+        if (m == 15u) {
+          sq[0] = p.s_table[nn];
+        } else {
+          sq[0] = (nn == 0 ? p.smin : p.s_table[nn - 1]) + dsf;
+          for (unsigned q = 0; q < m; ++q) sq[0] += dsf;
+        }
+        double f32[1];
+        unsigned amb;
+        unsigned long long unused = 0;
+        eval_along<PREC_GUARDED, 1, false, true>(p, ro, rd, sq, 1u, f32, unused, unused, cl,
+                                                 amb);
+        const double ev = density64(p, ro[0] + rd[0] * sq[0], ro[1] + rd[1] * sq[0],
+                                    ro[2] + rd[2] * sq[0], cl.idx, cl.n);
+        wrong = ev != f32[0];
+      }
+      for (unsigned long long w = __ballot(wrong); w; w &= w - 1ull) {
+        const int l = (int)__builtin_ctzll(w);
+        bad |= 1ull << __builtin_amdgcn_readlane(o, l);
+      }
+    }
+    if (cnt == 0u) continue;
+    const bool same = ((bad >> threadIdx.x) & 1ull) == 0ull;
     if (same) {
       reeval += cnt;
       ++rays;
