@@ -348,36 +348,81 @@ march_hier_redo_kernel(MarchParams p) {
   flush_counts(p.stats, rc);
 }
 
-// Resolve pass: one lane per entry of the first pass's lists (grid-stride
-// over min(counts[1], defer_cap) entries).  The deferred samples are
-// evaluated in FP64 and their terms summed in sample order into Ta, as the
-// one-pass march sums its re-evaluated terms; the pixel is T + Ta.  A wave
-// whose entries did not all fit wrote none of them (its rays are in the
-// redo list), and entries past the cap are not read.
-// 4 waves per SIMD: on C5 2.37 ms at 3 waves, 2.21 ms at 4.
+// Resolve pass: 64 entries of the first pass's lists per wave (grid-stride
+// over min(counts[1], defer_cap) entries), their deferred samples spread
+// one per lane.  Each is evaluated in FP64, its term (rho * dm) * ds staged
+// in LDS, and each entry's lane adds its own terms in sample order into Ta,
+// as the one-pass march sums its re-evaluated terms; the pixel is T + Ta.
+// A wave whose entries did not all fit wrote none of them but their count
+// 0 (its rays are in the redo list), and entries past the cap are not read.
+// 4 waves per SIMD (one entry per lane: C5 2.37 ms at 3 waves, 2.21 ms at
+// 4; lists of 1-24 samples).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
 march_resolve_kernel(MarchParams p) {
+  __shared__ double terms[64];
   const unsigned n = min(p.counts[1], p.defer_cap), nk = min(p.counts[2], p.k_cap);
   const unsigned rr = (unsigned)p.res * (unsigned)p.res;
   const double ds = p.ds, dm = p.dm;
   unsigned long long reeval = 0, rays = 0;
-  for (unsigned e = blockIdx.x * 64u + threadIdx.x; e < n; e += gridDim.x * 64u) {
-    const unsigned cnt = p.defer_cnt[e], k0 = p.defer_kfirst[e];
-    if (cnt == 0u || k0 + cnt > nk) continue;
-    const unsigned id = p.defer_ray[e];
+  for (unsigned base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+    const unsigned e = base + threadIdx.x;
+    unsigned cnt = 0u, k0 = 0u, id = 0u;
+    if (e < n) {
+      cnt = p.defer_cnt[e];
+      k0 = p.defer_kfirst[e];
+      if (k0 + cnt > nk) cnt = 0u;
+      // an entry marked empty (cnt 0, lists full) has no ray or k0 written
+      if (cnt != 0u)
+        id = p.defer_ray[e];
+      else
+        k0 = 0u;
+    }
     const unsigned v = id / rr, pix = id - v * rr;
     const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
     const Ray r = make_ray(p, p.views[p.view_offset + (int)v], i, j);
-    double Ta = 0.0;
-    for (unsigned c = 0; c < cnt; ++c) {
-      const double sk = p.s_table[p.defer_k[k0 + c]];
-      const double ev = density64(p, r.o[0] + r.d[0] * sk, r.o[1] + r.d[1] * sk,
-                                  r.o[2] + r.d[2] * sk);
-      Ta += (ev * dm) * ds;
+    // sample t of the concatenated lists belongs to the lane o whose
+    // inclusive prefix sum of counts is the first above t
+    unsigned incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned u = __shfl_up(incl, off, 64);
+      if ((int)threadIdx.x >= off) incl += u;
     }
-    store_pixel(p, (int)v, i, j, p.defer_T[e] + Ta);
-    reeval += cnt;
-    ++rays;
+    const unsigned excl = incl - cnt;
+    const unsigned total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    double Ta = 0.0;
+    for (unsigned fb = 0; fb < total; fb += 64u) {
+      const unsigned t = fb + threadIdx.x;
+      int o = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1)
+        if (__shfl(incl, o + step - 1, 64) <= t) o += step;
+      const unsigned c = t - __shfl(excl, o, 64);
+      const unsigned ko = __shfl(k0, o, 64);
+      double ro[3], rd[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        ro[q] = __shfl(r.o[q], o, 64);
+        rd[q] = __shfl(r.d[q], o, 64);
+      }
+      double term = 0.0;
+      if (t < total) {
+        const double sk = p.s_table[p.defer_k[ko + c]];
+        const double ev = density64(p, ro[0] + rd[0] * sk, ro[1] + rd[1] * sk,
+                                    ro[2] + rd[2] * sk);
+        term = (ev * dm) * ds;
+      }
+      terms[threadIdx.x] = term;
+      __syncthreads();
+      const unsigned lo = excl > fb ? excl : fb, hi = incl < fb + 64u ? incl : fb + 64u;
+      for (unsigned x = lo; x < hi; ++x) Ta += terms[x - fb];
+      __syncthreads();
+    }
+    if (cnt != 0u) {
+      store_pixel(p, (int)v, i, j, p.defer_T[e] + Ta);
+      reeval += cnt;
+      ++rays;
+    }
   }
   if (p.stats) {
     const unsigned long long a = wave_sum(reeval), b = wave_sum(rays);
