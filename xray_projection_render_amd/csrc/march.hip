@@ -221,8 +221,9 @@ march_hier_try_kernel(MarchParams p) {
 // culls the root collection's children to its own 64 rays first.
 // One listed sample per lane rather than one ray per lane: on C5 18.6 ->
 // 11.3 ms (a wave ran as many rounds as its longest list, 1-14 samples).
-// 4 waves per SIMD (128 VGPRs, the FP64 evaluator's spilled): 24.5 ms at 2
-// waves, 18.5 ms at 4, 22.2 ms at 5, one ray per lane.
+// 4 waves per SIMD (128 VGPRs, the FP64 evaluator's spilled): 11.2 ms, 11.4
+// at 3, 13.9 at 5 (one ray per lane: 24.5 ms at 2 waves, 18.5 at 4, 22.2
+// at 5).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
 march_hier_check_kernel(MarchParams p) {
   const unsigned n = min(p.counts[1], p.defer_cap), nk = min(p.counts[2], p.k_cap);
@@ -355,9 +356,9 @@ march_hier_redo_kernel(MarchParams p) {
 // as the one-pass march sums its re-evaluated terms; the pixel is T + Ta.
 // A wave whose entries did not all fit wrote none of them but their count
 // 0 (its rays are in the redo list), and entries past the cap are not read.
-// 4 waves per SIMD (one entry per lane: C5 2.37 ms at 3 waves, 2.21 ms at
-// 4; lists of 1-24 samples).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void
+// 3 waves per SIMD: on C5 1.57 ms, 1.65 ms at 4, 2.80 ms at 6 (one entry
+// per lane, lists of 1-24 samples: 2.37 ms at 3 waves, 2.21 ms at 4).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void
 march_resolve_kernel(MarchParams p) {
   __shared__ double terms[64];
   const unsigned n = min(p.counts[1], p.defer_cap), nk = min(p.counts[2], p.k_cap);
