@@ -354,6 +354,7 @@ march_hier_redo_kernel(MarchParams p) {
 // one per lane.  Each is evaluated in FP64, its term (rho * dm) * ds staged
 // in LDS, and each entry's lane adds its own terms in sample order into Ta,
 // as the one-pass march sums its re-evaluated terms; the pixel is T + Ta.
+// The FP64 walks of a root collection visit the wave's own cull list.
 // A wave whose entries did not all fit wrote none of them but their count
 // 0 (its rays are in the redo list), and entries past the cap are not read.
 // 3 waves per SIMD: on C5 1.57 ms, 1.65 ms at 4, 2.80 ms at 6 (one entry
@@ -380,6 +381,8 @@ march_resolve_kernel(MarchParams p) {
     }
     const unsigned v = id / rr, pix = id - v * rr;
     const int i = (int)(pix / (unsigned)p.res), j = (int)(pix - (unsigned)i * p.res);
+    // this wave's own cull list over its rays (a culled child is 0 along them)
+    const Cull cl = wave_cull<1>(p, (int)v, i, j, cnt != 0u);
     const Ray r = make_ray(p, p.views[p.view_offset + (int)v], i, j);
     // sample t of the concatenated lists belongs to the lane o whose
     // inclusive prefix sum of counts is the first above t
@@ -410,7 +413,7 @@ march_resolve_kernel(MarchParams p) {
       if (t < total) {
         const double sk = p.s_table[p.defer_k[ko + c]];
         const double ev = density64(p, ro[0] + rd[0] * sk, ro[1] + rd[1] * sk,
-                                    ro[2] + rd[2] * sk);
+                                    ro[2] + rd[2] * sk, cl.idx, cl.n);
         term = (ev * dm) * ds;
       }
       terms[threadIdx.x] = term;
